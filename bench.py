@@ -1,0 +1,178 @@
+"""Benchmark of the vlp4d spline time step on one MI355X.
+
+Workload: the reference's wk/SLD10.dat case at 128^4 (Landau damping,
+idcase 10, x,y in [0, 12.56637112), v in [-9, 9], dt = 0.01), synthetic
+initial data generated on the GPU.  A step is one onetimestep
+(miniapps/vlp4d_mpi/openmp/Timestep.hpp:34-97) including both Poisson solves
+and the diagnostics -- the reference's MainLoop timer scope.
+
+Metric: 4-D cell updates per second = N^4 * steps / wall time of the timed
+steps (inputs resident in HBM).  Prints ONE JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision 64|32]
+                  [--n 128] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+METRIC = "4D cell updates/s at 128^4 (FP32-store vs FP64 GPU) + % of HBM roofline"
+HBM_PEAK_GBS = 8000.0
+A100_REF = 2.38e9  # 40*128^4 / 4.51995 s, A100 stdpar MainLoop (docs/vlp4d.md:148,168-171)
+SLD10 = dict(minphy=[0.0, 0.0, -9.0, -9.0], maxphy=[12.56637112, 12.56637112, 9.0, 9.0], idcase=10, dt=0.01)
+
+DECK_TEMPLATE = """** Definition of mesh
+Number of points in Ex with the coarse mesh : {n}
+Number of points in Ey with the coarse mesh : {n}
+Number of points in Vx with the coarse mesh : {n}
+Number of points in Vy with the coarse mesh : {n}
+
+** Defintion of the geometry of the domain
+Minimal value of Ex : 0.
+Maximal value of Ex : 12.56637112
+Minimal value of Ey : 0.
+Maximal value of Ey : 12.56637112
+
+Minimal value of Vx : -9.
+Maximal value of Vx :  9.
+Minimal value of Vy : -9.
+Maximal value of Vy :  9.
+
+** Considered test cases
+ -10- Landau Damping
+ -11- Landau Damping 2
+ -20- Two beam instability
+Number of the chosen test case : 10
+
+** Iterations in time and diagnostics
+Time step : 0.01
+Number of total iterations : {nbiter}
+Frequency of diagnostics : 8
+Diagnostics of fxvx : 0
+"""
+
+
+def load_pkg():
+    import importlib.util
+    name = "p3_miniapps_amd"
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "p3-miniapps_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def algorithmic_bytes(n, b):
+    """Minimal HBM traffic of one step (bytes): read f, write CXY; read CXY
+    (advect2D); read CXY, write C4 (vx,vy splines); read C4, write f^{n+1}."""
+    e = n + 4
+    return b * (2 * n ** 4 + 3 * e * e * n * n + 2 * e ** 4)
+
+
+def advect4d_bytes(n, b):
+    """Algorithmic bytes of one advect_4D launch: read C4 once, write f^{n+1}."""
+    return b * ((n + 4) ** 4 + n ** 4)
+
+
+def cpu_baseline(n, steps=1):
+    """Oracle (C/OpenMP port of the reference OpenMP step) on the host cores,
+    one MainLoop step of the same deck.  Bounded sample: `steps` step(s)."""
+    exe = os.path.join(ROOT, "oracle", "vlp4d_oracle")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    cores = min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    with tempfile.TemporaryDirectory() as d:
+        deck = os.path.join(d, "bench.dat")
+        with open(deck, "w") as fh:
+            fh.write(DECK_TEMPLATE.format(n=n, nbiter=steps))
+        t0 = time.time()
+        out = subprocess.run([exe, "-f", deck], cwd=d, env=env, capture_output=True, text=True, check=True).stdout
+        wall = time.time() - t0
+    m = re.search(r"^MainLoop ([0-9.eE+-]+) \[s\], (\d+) calls", out, re.M)
+    sec = float(m.group(1))
+    return {"value": n ** 4 * steps / sec, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} MainLoop step(s) of the same 128^4 SLD10 deck, oracle/vlp4d_oracle "
+                      f"(FP64, OpenMP {cores} threads, {sec:.2f} s/step, {wall:.1f} s incl. init)" if n == 128 else
+                      f"{steps} step(s) at {n}^4, {sec:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", type=int, default=64, choices=[64, 32])
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "the multi-GPU decomposition is a design note, not an implemented path.", file=sys.stderr)
+        sys.exit(2)
+    pkg = load_pkg()
+    n = a.n
+    nb = a.warmup + a.steps
+    cfg = pkg.VlpConfig.make(n, SLD10["minphy"], SLD10["maxphy"], SLD10["idcase"], SLD10["dt"], nb)
+    prec = pkg.FP64 if a.precision == 64 else pkg.FP32
+    b = 8 if a.precision == 64 else 4
+    sim = pkg.Vlp4d(cfg, precision=prec)
+    if a.warmup:
+        sim.step(a.warmup)
+    t_before = sim.timers()
+    t0 = time.perf_counter()
+    sim.step(a.steps)
+    t1 = time.perf_counter()
+    t_after = sim.timers()
+    dt = t1 - t0
+    ms = 1e3 * dt / a.steps
+    value = n ** 4 * a.steps / dt
+    phases = {}
+    for lab in ("splinecoeff_xy", "advec2D", "field", "splinecoeff_vxvy", "advec4D"):
+        s = t_after[lab][0] - t_before[lab][0]
+        c = t_after[lab][1] - t_before[lab][1]
+        phases[lab] = round(1e3 * s / max(1, a.steps), 4)
+    adv_s = (t_after["advec4D"][0] - t_before["advec4D"][0]) / a.steps
+    adv_bytes = advect4d_bytes(n, b)
+    achieved = adv_bytes / adv_s / 1e9
+    d = sim.diags()
+    sim.close()
+    step_bytes = algorithmic_bytes(n, b)
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": (value / A100_REF) if a.precision == 64 and n == 128 else None,
+        "dtype": "f64" if a.precision == 64 else "f32-storage (f64 field solve)",
+        "data": "synthetic: SLD10 Landau-damping initial condition generated on the GPU",
+        "config": {"workload": f"vlp4d spline step, SLD10 deck at {n}^4 (box 12.56637112, v in [-9,9], dt 0.01)",
+                   "grid": [n, n, n, n], "idcase": 10, "dt": 0.01, "storage_bits": a.precision},
+        "roofline": {"bound": "hbm", "kernel": "k_advect4d", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": adv_bytes, "avg_launch_ms": 1e3 * adv_s},
+        "step_hbm_fraction": step_bytes / (dt / a.steps) / 1e9 / HBM_PEAK_GBS,
+        "step_algorithmic_bytes": step_bytes,
+        "phases_ms": phases,
+        "final_nrj": float(d["nrj"][-1]),
+    }
+    if not a.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(n)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,154 @@
+/*
+ * vlp4d.h -- C-ABI of the MI355X vlp4d time-step library (libvlp4d.so).
+ *
+ * The reference (miniapps/vlp4d_mpi) has no FFI: its hot path sits behind C++
+ * free functions called from main.  Each entry point below names the
+ * reference function it replaces (paths relative to the reference root;
+ * VMO/ = miniapps/vlp4d_mpi/openmp/).  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Every function returns VLP_OK (0) or a VLP_E_* code and never exits;
+ *    vlp_last_error() gives the message.  The reference's exit()/abort()
+ *    behaviour is reproduced by the command-line driver only.
+ *  - One context = one device + one HIP stream.  Calls on a context must be
+ *    serialised by the caller (as the reference's single host thread does).
+ *  - 4-D host arrays are the interior N^4 of the distribution function in the
+ *    reference's order: x fastest, then y, vx, vy (layout_left,
+ *    lib/openmp/View.hpp:52-451), always as double on the host side.
+ *  - 2-D host arrays (rho/phi, Ex, Ey) are Nx*Ny, x fastest.
+ */
+#ifndef VLP4D_H
+#define VLP4D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  VLP_OK = 0,
+  VLP_E_FILE = 1,   /* deck missing/unreadable: reference abort(), VMO/Init.cpp:18-21        */
+  VLP_E_CASE = 2,   /* unknown idcase: reference abort(), VMO/Init.cpp:142-145               */
+  VLP_E_CFL = 3,    /* foot outside domain: reference "Time step is too large" + exit(0),
+                       VMO/Advection.hpp:18-23                                              */
+  VLP_E_ARG = 4,    /* invalid argument / state                                             */
+  VLP_E_HIP = 5,    /* HIP runtime error (no device, out of memory, launch failure)          */
+  VLP_E_SIZE = 6    /* N_d < MMAX+2 = 10 or N_d+5 > 256 (VMO/Init.cpp:85-89,
+                       VMO/Types.hpp:31), or buffer size mismatch                           */
+};
+
+/* Storage precision of the distribution function and spline coefficients. */
+enum { VLP_FP64 = 64, VLP_FP32 = 32 };
+
+/* Domain of VMO/Config.hpp:34-48 for one rank (dx derived as (max-min)/N). */
+typedef struct {
+  int nxmax[4];       /* points in x, y, vx, vy                         */
+  double minPhy[4];
+  double maxPhy[4];
+  double dx[4];       /* filled by the library                          */
+  int idcase;         /* 2, 5, 10, 20 (VMO/Init.cpp:129-146)             */
+  double dt;
+  int nbiter;
+  int ifreq;
+  int fxvx;
+} vlp_config;
+
+typedef struct vlp_ctx vlp_ctx;
+
+/* Phase identifiers for vlp_timer(): the labels of miniapps/vlp4d_mpi/Timer.hpp:37-66. */
+enum {
+  VLP_T_TOTAL = 0, VLP_T_MAINLOOP, VLP_T_PACK, VLP_T_COMM, VLP_T_UNPACK,
+  VLP_T_ADVEC2D, VLP_T_ADVEC4D, VLP_T_FIELD, VLP_T_ALLREDUCE, VLP_T_FOURIER,
+  VLP_T_DIAG, VLP_T_SPLINE_XY, VLP_T_SPLINE_VXVY, VLP_T_COUNT
+};
+
+/* Device buffers exposed for phase-level parity checks (vlp_get_buffer). */
+enum {
+  VLP_BUF_F = 0,      /* f^n interior, N^4                                            */
+  VLP_BUF_CXY = 1,    /* spline coeffs along x,y: x,y in [-2,N+1], vx,vy interior     */
+  VLP_BUF_C4 = 2,     /* spline coeffs along all 4 dims on [-2,N+1]^4                  */
+  VLP_BUF_RHO = 3,    /* after a field solve: phi (the reference overwrites rho with phi) */
+  VLP_BUF_EX = 4,
+  VLP_BUF_EY = 5,
+  VLP_BUF_COUNT = 6
+};
+
+/* Library version string. */
+const char *vlp_version(void);
+
+/* Parse a .dat deck with the reference's semantics (VMO/Init.cpp:12-90 `import`).
+ * Pure host code; needs no GPU. */
+int vlp_parse_deck(const char *dat_path, vlp_config *cfg);
+
+/* init() of VMO/Init.cpp:324-373 plus the initial field solve and diagnostics
+ * of VMO/Vlp4d.cpp:29-34: allocates device state on `device`, runs the init
+ * case, solves Poisson, records diagnostics at iter 0.  precision: VLP_FP64 or
+ * VLP_FP32.  dat_path variant parses the deck first. */
+int vlp_create(const char *dat_path, int precision, int device, vlp_ctx **out);
+int vlp_create_config(const vlp_config *cfg, int precision, int device, vlp_ctx **out);
+
+/* finalize() of VMO/Init.cpp:375-382 (without the file output; see vlp_save). */
+int vlp_destroy(vlp_ctx *ctx);
+
+/* `nsteps` iterations of onetimestep (VMO/Timestep.hpp:34-97) followed by
+ * fn.swap(fnp1) (VMO/Vlp4d.cpp:137-147).  Returns VLP_E_CFL if a foot left the
+ * domain (state is then invalid). */
+int vlp_step(vlp_ctx *ctx, int nsteps);
+
+/* Current iteration count (the reference's `iter`). */
+int vlp_iter(const vlp_ctx *ctx);
+
+/* Copy of the parsed/derived configuration. */
+int vlp_get_config(const vlp_ctx *ctx, vlp_config *cfg);
+
+/* Diagnostic series of Diags (VMO/Diags.cpp:16-76) for iterations 0..n-1
+ * (n <= iter+1).  Any pointer may be NULL. */
+int vlp_diags(vlp_ctx *ctx, double *nrj, double *nrjx, double *nrjy, double *mass, double *l2norm, int n);
+
+/* Diags::save (VMO/Diags.cpp:78-104): writes dir/nrj.out and dir/l2norm.out
+ * in the reference's format for iterations 0..nbiter. */
+int vlp_save(vlp_ctx *ctx, const char *dir);
+
+/* Interior f^n as double, n = Nx*Ny*Nvx*Nvy (View::updateSelf / updateDevice,
+ * lib/openmp/View.hpp:225-235).  vlp_set_f also recomputes rho, E and the
+ * iteration-`iter` diagnostics from the new f, like VMO/Vlp4d.cpp:30-34. */
+int vlp_get_f(vlp_ctx *ctx, double *host, size_t n);
+int vlp_set_f(vlp_ctx *ctx, const double *host, size_t n);
+
+/* Copy an internal buffer (VLP_BUF_*) to the host as double; n must equal its
+ * element count (vlp_buffer_size). */
+size_t vlp_buffer_size(const vlp_ctx *ctx, int which);
+int vlp_get_buffer(vlp_ctx *ctx, int which, double *host, size_t n);
+
+/* ---- phase entry points (the reference's component functions) ----------- */
+
+/* exchangeHalo + Spline::computeCoeff_xy (VMO/Communication.cpp:541-570,
+ * VMO/Spline.hpp:384-426): f^n -> CXY. */
+int vlp_spline_xy(vlp_ctx *ctx);
+
+/* Advection::advect_2D_xy with dt/2, field_rho, field_reduce, field_poisson
+ * (VMO/Advection.hpp:299-406, VMO/Field.cpp:5-137): CXY -> rho, E, phi. */
+int vlp_advect2d_poisson(vlp_ctx *ctx);
+
+/* Spline::computeCoeff_vxvy (VMO/Spline.hpp:429-469): CXY -> C4. */
+int vlp_spline_vxvy(vlp_ctx *ctx);
+
+/* Advection::advect_4D (VMO/Advection.hpp:409-498) then field_rho/reduce/
+ * poisson, Diags::compute and computeL2norm (VMO/Timestep.hpp:73-91):
+ * C4, E -> f^{n+1}, written into F; iteration counter advanced. */
+int vlp_advect4d_poisson(vlp_ctx *ctx);
+
+/* Timer table of miniapps/vlp4d_mpi/Timer.hpp:70-75: seconds and calls for
+ * VLP_T_* (GPU phases measured with HIP events). */
+int vlp_timer(vlp_ctx *ctx, int which, double *seconds, int *calls);
+const char *vlp_timer_label(int which);
+
+/* Message of the last error on this context (or the last global error when
+ * ctx is NULL). */
+const char *vlp_last_error(const vlp_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

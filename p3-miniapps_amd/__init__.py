@@ -1,0 +1,231 @@
+"""Python mirror of the vlp4d driver interface over the C-ABI of libvlp4d.so.
+
+The reference (miniapps/vlp4d_mpi/openmp) drives its hot path from C++:
+``init`` (VMO/Init.cpp:324), ``onetimestep`` (VMO/Timestep.hpp:34), the
+component functions ``Spline::computeCoeff_xy/vxvy``, ``Advection::advect_2D_xy
+/advect_4D``, ``field_rho/field_reduce/field_poisson`` and ``Diags::compute/
+computeL2norm/save``.  :class:`Vlp4d` exposes the same operations on one GPU:
+the whole step (:meth:`Vlp4d.step`) and the four phases of one step.
+
+Errors are raised as :class:`Vlp4dError` carrying the C-ABI status code; the
+reference's ``exit(0)`` on a too-large time step is :data:`E_CFL` here.
+
+The directory name is not a Python identifier; import it with
+``importlib`` (see ``__graft_entry__.py``) or load ``libvlp4d.so`` directly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvlp4d.so")
+
+OK, E_FILE, E_CASE, E_CFL, E_ARG, E_HIP, E_SIZE = range(7)
+FP64, FP32 = 64, 32
+BUF_F, BUF_CXY, BUF_C4, BUF_RHO, BUF_EX, BUF_EY = range(6)
+TIMER_LABELS = ["total", "MainLoop", "pack", "comm", "unpack", "advec2D", "advec4D", "field",
+                "all_reduce", "Fourier", "diag", "splinecoeff_xy", "splinecoeff_vxvy"]
+
+# Every symbol declared in include/vlp4d.h.
+EXPORTS = [
+    "vlp_version", "vlp_parse_deck", "vlp_create", "vlp_create_config", "vlp_destroy", "vlp_step",
+    "vlp_iter", "vlp_get_config", "vlp_diags", "vlp_save", "vlp_get_f", "vlp_set_f",
+    "vlp_buffer_size", "vlp_get_buffer", "vlp_spline_xy", "vlp_advect2d_poisson",
+    "vlp_spline_vxvy", "vlp_advect4d_poisson", "vlp_timer", "vlp_timer_label", "vlp_last_error",
+]
+
+
+class VlpConfig(ctypes.Structure):
+    """``vlp_config`` of include/vlp4d.h (Domain of VMO/Config.hpp:34-48)."""
+    _fields_ = [
+        ("nxmax", ctypes.c_int * 4),
+        ("minPhy", ctypes.c_double * 4),
+        ("maxPhy", ctypes.c_double * 4),
+        ("dx", ctypes.c_double * 4),
+        ("idcase", ctypes.c_int),
+        ("dt", ctypes.c_double),
+        ("nbiter", ctypes.c_int),
+        ("ifreq", ctypes.c_int),
+        ("fxvx", ctypes.c_int),
+    ]
+
+    @classmethod
+    def make(cls, n, minphy, maxphy, idcase, dt, nbiter, ifreq=1, fxvx=0):
+        c = cls()
+        n = list(n) if hasattr(n, "__len__") else [n] * 4
+        for i in range(4):
+            c.nxmax[i] = int(n[i])
+            c.minPhy[i] = float(minphy[i])
+            c.maxPhy[i] = float(maxphy[i])
+            c.dx[i] = (c.maxPhy[i] - c.minPhy[i]) / c.nxmax[i]
+        c.idcase, c.dt, c.nbiter, c.ifreq, c.fxvx = int(idcase), float(dt), int(nbiter), int(ifreq), int(fxvx)
+        return c
+
+    def as_dict(self):
+        return {"nxmax": list(self.nxmax), "minPhy": list(self.minPhy), "maxPhy": list(self.maxPhy),
+                "dx": list(self.dx), "idcase": self.idcase, "dt": self.dt, "nbiter": self.nbiter,
+                "ifreq": self.ifreq, "fxvx": self.fxvx}
+
+
+class Vlp4dError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vlp4d error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libvlp4d.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built; run p3-miniapps_amd/build.py")
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    dp = P(ctypes.c_double)
+    lib.vlp_version.restype = ctypes.c_char_p
+    lib.vlp_parse_deck.argtypes = [ctypes.c_char_p, P(VlpConfig)]
+    lib.vlp_create.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P(vp)]
+    lib.vlp_create_config.argtypes = [P(VlpConfig), ctypes.c_int, ctypes.c_int, P(vp)]
+    lib.vlp_destroy.argtypes = [vp]
+    lib.vlp_step.argtypes = [vp, ctypes.c_int]
+    lib.vlp_iter.argtypes = [vp]
+    lib.vlp_get_config.argtypes = [vp, P(VlpConfig)]
+    lib.vlp_diags.argtypes = [vp, dp, dp, dp, dp, dp, ctypes.c_int]
+    lib.vlp_save.argtypes = [vp, ctypes.c_char_p]
+    lib.vlp_get_f.argtypes = [vp, dp, ctypes.c_size_t]
+    lib.vlp_set_f.argtypes = [vp, dp, ctypes.c_size_t]
+    lib.vlp_buffer_size.argtypes = [vp, ctypes.c_int]
+    lib.vlp_buffer_size.restype = ctypes.c_size_t
+    lib.vlp_get_buffer.argtypes = [vp, ctypes.c_int, dp, ctypes.c_size_t]
+    for name in ("vlp_spline_xy", "vlp_advect2d_poisson", "vlp_spline_vxvy", "vlp_advect4d_poisson"):
+        getattr(lib, name).argtypes = [vp]
+    lib.vlp_timer.argtypes = [vp, ctypes.c_int, dp, P(ctypes.c_int)]
+    lib.vlp_timer_label.argtypes = [ctypes.c_int]
+    lib.vlp_timer_label.restype = ctypes.c_char_p
+    lib.vlp_last_error.argtypes = [vp]
+    lib.vlp_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def parse_deck(path):
+    """``import`` of VMO/Init.cpp:12-90."""
+    lib = load()
+    cfg = VlpConfig()
+    rc = lib.vlp_parse_deck(os.fsencode(path), ctypes.byref(cfg))
+    if rc:
+        raise Vlp4dError(rc, lib.vlp_last_error(None).decode())
+    return cfg
+
+
+class Vlp4d:
+    """One simulation on one GPU (context of the C-ABI)."""
+
+    def __init__(self, cfg_or_path, precision=FP64, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        if isinstance(cfg_or_path, (str, bytes, os.PathLike)):
+            rc = self.lib.vlp_create(os.fsencode(cfg_or_path), precision, device, ctypes.byref(h))
+        else:
+            rc = self.lib.vlp_create_config(ctypes.byref(cfg_or_path), precision, device, ctypes.byref(h))
+        if rc:
+            raise Vlp4dError(rc, self.lib.vlp_last_error(None).decode())
+        self.h = h
+        self.precision = precision
+        self.cfg = VlpConfig()
+        self.lib.vlp_get_config(self.h, ctypes.byref(self.cfg))
+        self.shape4 = tuple(self.cfg.nxmax[i] for i in range(4))
+
+    def _check(self, rc):
+        if rc:
+            raise Vlp4dError(rc, self.lib.vlp_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vlp_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def iter(self):
+        return self.lib.vlp_iter(self.h)
+
+    def step(self, n=1):
+        self._check(self.lib.vlp_step(self.h, n))
+
+    # phases of onetimestep (VMO/Timestep.hpp:41-96)
+    def spline_xy(self):
+        self._check(self.lib.vlp_spline_xy(self.h))
+
+    def advect2d_poisson(self):
+        self._check(self.lib.vlp_advect2d_poisson(self.h))
+
+    def spline_vxvy(self):
+        self._check(self.lib.vlp_spline_vxvy(self.h))
+
+    def advect4d_poisson(self):
+        self._check(self.lib.vlp_advect4d_poisson(self.h))
+
+    def get_f(self):
+        """Interior f (x fastest) as an array of shape (Nvy, Nvx, Ny, Nx)."""
+        n = int(np.prod(self.shape4))
+        out = np.empty(n, dtype=np.float64)
+        self._check(self.lib.vlp_get_f(self.h, _dptr(out), n))
+        return out.reshape(self.shape4[::-1])
+
+    def set_f(self, f):
+        a = np.ascontiguousarray(f, dtype=np.float64).ravel()
+        self._check(self.lib.vlp_set_f(self.h, _dptr(a), a.size))
+
+    def buffer(self, which):
+        n = self.lib.vlp_buffer_size(self.h, which)
+        out = np.empty(n, dtype=np.float64)
+        self._check(self.lib.vlp_get_buffer(self.h, which, _dptr(out), n))
+        N = self.shape4
+        if which == BUF_CXY:
+            return out.reshape(N[3], N[2], N[1] + 4, N[0] + 4)
+        if which == BUF_C4:
+            return out.reshape(N[3] + 4, N[2] + 4, N[1] + 4, N[0] + 4)
+        if which == BUF_F:
+            return out.reshape(N[::-1])
+        return out.reshape(N[1], N[0])
+
+    def diags(self, n=None):
+        """dict of nrj, nrjx, nrjy, mass, l2norm for iterations 0..n-1."""
+        n = self.iter + 1 if n is None else n
+        arrs = [np.zeros(n) for _ in range(5)]
+        self._check(self.lib.vlp_diags(self.h, *[_dptr(a) for a in arrs], n))
+        return dict(zip(["nrj", "nrjx", "nrjy", "mass", "l2norm"], arrs))
+
+    def save(self, directory="."):
+        self._check(self.lib.vlp_save(self.h, os.fsencode(directory)))
+
+    def timers(self):
+        out = {}
+        for k, lab in enumerate(TIMER_LABELS):
+            s = ctypes.c_double()
+            c = ctypes.c_int()
+            self.lib.vlp_timer(self.h, k, ctypes.byref(s), ctypes.byref(c))
+            out[lab] = (s.value, c.value)
+        return out

@@ -1,0 +1,594 @@
+// vlp4d_api.cpp -- C-ABI of libvlp4d.so: context, time step orchestration,
+// deck parser, diagnostics and timers (host side).
+//
+// The step is the reference's onetimestep (miniapps/vlp4d_mpi/openmp/
+// Timestep.hpp:34-97) re-planned for one GPU as four streaming sweeps over
+// HBM, each a hand-written kernel (kernels.hip):
+//   1. spline x, spline y      f     -> CXY   (halo + alpha-sums computed in-kernel)
+//   2. advect 2-D + rho        CXY   -> rho partials (f_half never stored)
+//   3. Poisson + E             rho   -> E, phi
+//   4. spline vx, spline vy    CXY   -> C4
+//   5. advect 4-D + rho + L2   C4, E -> f^{n+1}, rho partials
+//   6. Poisson + diagnostics
+// The reference's deep copies, transposes and halo buffers have no counterpart.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vlp4d.h"
+#include "vlp4d_internal.hpp"
+
+using vlp::FieldBufs;
+using vlp::Geom;
+
+static thread_local std::string g_last_error;
+
+struct vlp_ctx {
+  vlp_config cfg{};
+  Geom g{};
+  int precision = VLP_FP64;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t esz = 8;
+  void *f = nullptr, *fnext = nullptr, *cxy = nullptr, *tv = nullptr, *c4 = nullptr;
+  size_t n_f = 0, n_cxy = 0, n_tv = 0, n_c4 = 0;
+  double *rho_part = nullptr, *l2_part = nullptr;
+  long n_l2 = 0;
+  FieldBufs fb{};
+  int *d_err = nullptr;
+  int *h_err = nullptr;
+  int iter = 0;
+  bool broken = false;
+  std::string err;
+  double tsec[VLP_T_COUNT] = {};
+  int tcalls[VLP_T_COUNT] = {};
+  hipEvent_t ev[8] = {};
+};
+
+namespace {
+
+const char *kLabels[VLP_T_COUNT] = {"total",      "MainLoop", "pack",    "comm", "unpack",
+                                    "advec2D",    "advec4D",  "field",   "all_reduce",
+                                    "Fourier",    "diag",     "splinecoeff_xy", "splinecoeff_vxvy"};
+
+int set_err(vlp_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(c, expr)                                                                           \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return set_err((c), VLP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+// Advance past the next ':' (VMO/Init.cpp:33-35); EOF is an error here
+// where the reference would spin forever.
+bool skip_to_colon(FILE *s) {
+  int ch;
+  do {
+    ch = fgetc(s);
+    if (ch == EOF) return false;
+  } while (ch != ':');
+  return true;
+}
+
+#pragma clang fp contract(off)
+void derive_dx(vlp_config *c) {
+  for (int i = 0; i < 4; i++) c->dx[i] = (c->maxPhy[i] - c->minPhy[i]) / (double)c->nxmax[i];  // VMO/Init.cpp:80-82
+}
+#pragma clang fp contract(on)
+
+int check_config(const vlp_config *c) {
+  for (int i = 0; i < 4; i++) {
+    if (c->nxmax[i] < 10) return set_err(nullptr, VLP_E_SIZE, "dimension smaller than MMAX+2 = 10");
+    if (c->nxmax[i] + 5 > 256) return set_err(nullptr, VLP_E_SIZE, "dimension larger than 251 (spline line buffer)");
+  }
+  if (c->nbiter < 0 || c->ifreq <= 0) return set_err(nullptr, VLP_E_ARG, "nbiter < 0 or ifreq <= 0");
+  if (c->idcase != 2 && c->idcase != 5 && c->idcase != 10 && c->idcase != 20)
+    return set_err(nullptr, VLP_E_CASE, "Unknown test case !");
+  if (!(c->dt > 0) || !std::isfinite(c->dt)) return set_err(nullptr, VLP_E_ARG, "dt must be positive");
+  for (int i = 0; i < 4; i++)
+    if (!(c->maxPhy[i] > c->minPhy[i])) return set_err(nullptr, VLP_E_ARG, "maxPhy must exceed minPhy");
+  return VLP_OK;
+}
+
+// exp(-2 pi i m / n) with exact values on the axes, interleaved complex.
+std::vector<double> twiddles(int n) {
+  std::vector<double> tw(2 * (size_t)n);
+  for (int m = 0; m < n; m++) {
+    double re, im;
+    if (m == 0) { re = 1.0; im = 0.0; }
+    else if (2 * m == n) { re = -1.0; im = 0.0; }
+    else if (4 * m == n) { re = 0.0; im = -1.0; }
+    else if (4 * m == 3 * n) { re = 0.0; im = 1.0; }
+    else {
+      const double ang = 2.0 * M_PI * (double)m / (double)n;
+      re = std::cos(ang);
+      im = -std::sin(ang);
+    }
+    tw[2 * m] = re;
+    tw[2 * m + 1] = im;
+  }
+  return tw;
+}
+
+void free_all(vlp_ctx *c) {
+  void *bufs[] = {c->f,        c->fnext,    c->cxy,       c->tv,     c->c4,      c->rho_part, c->l2_part,
+                  c->fb.rho,   c->fb.ex,    c->fb.ey,     c->fb.spec1, c->fb.spec2, c->fb.twx,  c->fb.twy,
+                  c->fb.diag_part, c->fb.diag, c->d_err};
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
+  if (c->h_err) (void)hipHostFree(c->h_err);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+template <typename T> int run_spline_xy(vlp_ctx *c) {
+  const Geom &g = c->g;
+  const long N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const long E0 = N0 + 4, E1 = N1 + 4;
+  T *tx = (T *)c->tv;
+  vlp::launch_spline_x<T>(g, (const T *)c->f, tx, c->stream);
+  // y-pass: tx (E0, N1, N2*N3) -> cxy (E0, E1, N2*N3)
+  vlp::launch_spline_line<T>(tx, (T *)c->cxy, (int)N1, E0, N2 * N3, E0, E0 * N1, E0, E0 * E1, c->stream);
+  (void)N2;
+  (void)N3;
+  return VLP_OK;
+}
+
+template <typename T> int run_spline_vxvy(vlp_ctx *c) {
+  const Geom &g = c->g;
+  const long N2 = g.n[2], N3 = g.n[3];
+  const long P = (long)(g.n[0] + 4) * (g.n[1] + 4);
+  const long E2 = N2 + 4;
+  // vx-pass: cxy (P, N2, N3) -> tv (P, E2, N3)
+  vlp::launch_spline_line<T>((const T *)c->cxy, (T *)c->tv, (int)N2, P, N3, P, P * N2, P, P * E2, c->stream);
+  // vy-pass: tv (P*E2, N3) -> c4 (P*E2, E3)
+  vlp::launch_spline_line<T>((const T *)c->tv, (T *)c->c4, (int)N3, P * E2, 1, P * E2, 0, P * E2, 0, c->stream);
+  return VLP_OK;
+}
+
+template <typename T> void run_advect2d(vlp_ctx *c) {
+  vlp::launch_advect2d_rho<T>(c->g, (const T *)c->cxy, 0.5 * c->cfg.dt, c->rho_part, c->d_err, c->stream);
+}
+
+template <typename T> void run_advect4d(vlp_ctx *c) {
+  vlp::launch_advect4d<T>(c->g, (const T *)c->c4, c->fb.ex, c->fb.ey, c->cfg.dt, (T *)c->fnext, c->rho_part,
+                          c->l2_part, c->d_err, c->stream);
+}
+
+template <typename T> void run_moments(vlp_ctx *c) {
+  vlp::launch_moments<T>(c->g, (const T *)c->f, c->rho_part, c->l2_part, c->stream);
+}
+
+int check_launch(vlp_ctx *c, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    c->broken = true;
+    return set_err(c, VLP_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  }
+  return VLP_OK;
+}
+
+// device-side error flag -> VLP_E_CFL (reference: testError, VMO/Advection.hpp:18-23)
+int sync_and_check(vlp_ctx *c) {
+  HIP_TRY(c, hipMemcpyAsync(c->h_err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (*c->h_err) {
+    c->broken = true;
+    return set_err(c, VLP_E_CFL, "Time step is too large, exiting");
+  }
+  return VLP_OK;
+}
+
+int field(vlp_ctx *c, int iter, bool record, bool with_l2) {
+  vlp::launch_field(c->g, c->fb, c->rho_part, with_l2 ? c->l2_part : nullptr, c->n_l2, iter, record, c->stream);
+  return check_launch(c, "field");
+}
+
+float ev_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  return ms;
+}
+
+// One onetimestep (VMO/Timestep.hpp:34-97) + swap, events between phases.
+int step_once(vlp_ctx *c) {
+  const bool f64 = c->precision == VLP_FP64;
+  hipEvent_t *ev = c->ev;
+  HIP_TRY(c, hipEventRecord(ev[0], c->stream));
+  if (f64) run_spline_xy<double>(c); else run_spline_xy<float>(c);
+  HIP_TRY(c, hipEventRecord(ev[1], c->stream));
+  if (f64) run_advect2d<double>(c); else run_advect2d<float>(c);
+  HIP_TRY(c, hipEventRecord(ev[2], c->stream));
+  int rc = field(c, c->iter + 1, false, false);
+  if (rc) return rc;
+  HIP_TRY(c, hipEventRecord(ev[3], c->stream));
+  if (f64) run_spline_vxvy<double>(c); else run_spline_vxvy<float>(c);
+  HIP_TRY(c, hipEventRecord(ev[4], c->stream));
+  if (f64) run_advect4d<double>(c); else run_advect4d<float>(c);
+  HIP_TRY(c, hipEventRecord(ev[5], c->stream));
+  rc = field(c, c->iter + 1, true, true);
+  if (rc) return rc;
+  HIP_TRY(c, hipEventRecord(ev[6], c->stream));
+  rc = check_launch(c, "step");
+  if (rc) return rc;
+  std::swap(c->f, c->fnext);
+  c->iter++;
+  rc = sync_and_check(c);
+  if (rc) return rc;
+  c->tsec[VLP_T_SPLINE_XY] += 1e-3 * ev_ms(ev[0], ev[1]);
+  c->tsec[VLP_T_ADVEC2D] += 1e-3 * ev_ms(ev[1], ev[2]);
+  c->tsec[VLP_T_FIELD] += 1e-3 * (ev_ms(ev[2], ev[3]) + ev_ms(ev[5], ev[6]));
+  c->tsec[VLP_T_SPLINE_VXVY] += 1e-3 * ev_ms(ev[3], ev[4]);
+  c->tsec[VLP_T_ADVEC4D] += 1e-3 * ev_ms(ev[4], ev[5]);
+  c->tcalls[VLP_T_SPLINE_XY]++;
+  c->tcalls[VLP_T_ADVEC2D]++;
+  c->tcalls[VLP_T_ADVEC4D]++;
+  c->tcalls[VLP_T_SPLINE_VXVY]++;
+  c->tcalls[VLP_T_FIELD] += 2;
+  c->tcalls[VLP_T_ALLREDUCE] += 2;
+  c->tcalls[VLP_T_FOURIER] += 2;
+  c->tcalls[VLP_T_DIAG] += 2;
+  c->tcalls[VLP_T_PACK]++;
+  c->tcalls[VLP_T_COMM]++;
+  c->tcalls[VLP_T_UNPACK]++;
+  return VLP_OK;
+}
+
+int usable(vlp_ctx *c) {
+  if (!c) return set_err(nullptr, VLP_E_ARG, "null context");
+  if (c->broken) return set_err(c, VLP_E_ARG, "context unusable after an earlier error: " + c->err);
+  if (hipSetDevice(c->device) != hipSuccess) return set_err(c, VLP_E_HIP, "hipSetDevice failed");
+  return VLP_OK;
+}
+
+template <typename T> int upload_f(vlp_ctx *c, const double *host) {
+  std::vector<T> tmp;
+  const void *src = host;
+  if (sizeof(T) != sizeof(double)) {
+    tmp.resize(c->n_f);
+    for (size_t i = 0; i < c->n_f; i++) tmp[i] = (T)host[i];
+    src = tmp.data();
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->f, src, c->n_f * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return VLP_OK;
+}
+
+int download(vlp_ctx *c, const void *dev, size_t n, double *host) {
+  if (c->esz == sizeof(double)) {
+    HIP_TRY(c, hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(n);
+    HIP_TRY(c, hipMemcpyAsync(tmp.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; i++) host[i] = tmp[i];
+  }
+  return VLP_OK;
+}
+
+// field solve + diagnostics of the current f at iteration `iter`
+// (VMO/Vlp4d.cpp:30-34 for iter 0).
+int refresh_from_f(vlp_ctx *c) {
+  if (c->precision == VLP_FP64) run_moments<double>(c); else run_moments<float>(c);
+  int rc = field(c, c->iter, true, true);
+  if (rc) return rc;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return VLP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *vlp_version(void) { return "vlp4d-mi355x 0.1 (gfx950)"; }
+
+int vlp_parse_deck(const char *path, vlp_config *cfg) {
+  if (!path || !cfg) return set_err(nullptr, VLP_E_ARG, "null argument");
+  FILE *s = fopen(path, "r");
+  if (!s) return set_err(nullptr, VLP_E_FILE, "import: error file not found");
+  vlp_config c{};
+  char idcase[8] = {0};
+  bool ok = true;
+  for (int i = 0; i < 4 && ok; i++) {  // VMO/Init.cpp:32-39
+    int nx = 0;
+    ok = skip_to_colon(s) && fscanf(s, " %d\n", &nx) == 1;
+    c.nxmax[i] = nx;
+  }
+  for (int i = 0; i < 4 && ok; i++) {  // VMO/Init.cpp:41-51
+    ok = skip_to_colon(s) && fscanf(s, " %le\n", &c.minPhy[i]) == 1;
+    ok = ok && skip_to_colon(s) && fscanf(s, " %le\n", &c.maxPhy[i]) == 1;
+  }
+  ok = ok && skip_to_colon(s) && fgets(idcase, 7, s) != nullptr;   // VMO/Init.cpp:53-56
+  ok = ok && skip_to_colon(s) && fscanf(s, " %le\n", &c.dt) == 1;  // :58-61
+  ok = ok && skip_to_colon(s) && fscanf(s, " %d\n", &c.nbiter) == 1;
+  ok = ok && skip_to_colon(s) && fscanf(s, " %d\n", &c.ifreq) == 1;
+  ok = ok && skip_to_colon(s) && fscanf(s, " %d\n", &c.fxvx) == 1;
+  fclose(s);
+  if (!ok) return set_err(nullptr, VLP_E_FILE, "import: malformed deck");
+  c.idcase = atoi(idcase);  // VMO/Init.cpp:78
+  derive_dx(&c);
+  *cfg = c;
+  return VLP_OK;
+}
+
+int vlp_create(const char *dat_path, int precision, int device, vlp_ctx **out) {
+  vlp_config cfg;
+  const int rc = vlp_parse_deck(dat_path, &cfg);
+  if (rc) return rc;
+  return vlp_create_config(&cfg, precision, device, out);
+}
+
+int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_ctx **out) {
+  if (!cfg_in || !out) return set_err(nullptr, VLP_E_ARG, "null argument");
+  *out = nullptr;
+  if (precision != VLP_FP64 && precision != VLP_FP32) return set_err(nullptr, VLP_E_ARG, "precision must be 64 or 32");
+  vlp_config cfg = *cfg_in;
+  derive_dx(&cfg);
+  int rc = check_config(&cfg);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return set_err(nullptr, VLP_E_HIP, "no HIP device available");
+  if (device < 0 || device >= ndev) return set_err(nullptr, VLP_E_ARG, "device index out of range");
+  vlp_ctx *c = new vlp_ctx();
+  c->cfg = cfg;
+  c->precision = precision;
+  c->device = device;
+  c->esz = precision == VLP_FP64 ? sizeof(double) : sizeof(float);
+  for (int i = 0; i < 4; i++) {
+    c->g.n[i] = cfg.nxmax[i];
+    c->g.minPhy[i] = cfg.minPhy[i];
+    c->g.maxPhy[i] = cfg.maxPhy[i];
+    c->g.dx[i] = cfg.dx[i];
+    c->g.inv_dx[i] = 1. / cfg.dx[i];
+  }
+  c->g.idcase = cfg.idcase;
+  const size_t N0 = cfg.nxmax[0], N1 = cfg.nxmax[1], N2 = cfg.nxmax[2], N3 = cfg.nxmax[3];
+  c->n_f = N0 * N1 * N2 * N3;
+  c->n_cxy = (N0 + 4) * (N1 + 4) * N2 * N3;
+  c->n_tv = (N0 + 4) * (N1 + 4) * (N2 + 4) * N3;
+  c->n_c4 = (N0 + 4) * (N1 + 4) * (N2 + 4) * (N3 + 4);
+  c->n_l2 = vlp::l2_partials(c->g);
+  const size_t nxh = N0 / 2 + 1;
+  auto fail = [&](int code) {
+    free_all(c);
+    delete c;
+    return code;
+  };
+#define ALLOC(ptr, bytes)                                                                   \
+  do {                                                                                      \
+    if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess)                                  \
+      return fail(set_err(nullptr, VLP_E_HIP, std::string("hipMalloc failed: ") + #ptr));  \
+  } while (0)
+  if (hipSetDevice(device) != hipSuccess) return fail(set_err(nullptr, VLP_E_HIP, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_err(nullptr, VLP_E_HIP, "hipStreamCreate failed"));
+  for (auto &e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(set_err(nullptr, VLP_E_HIP, "hipEventCreate failed"));
+  ALLOC(c->f, c->n_f * c->esz);
+  ALLOC(c->fnext, c->n_f * c->esz);
+  ALLOC(c->cxy, c->n_cxy * c->esz);
+  ALLOC(c->tv, c->n_tv * c->esz);
+  ALLOC(c->c4, c->n_c4 * c->esz);
+  ALLOC(c->rho_part, N0 * N1 * N3 * sizeof(double));
+  ALLOC(c->l2_part, (size_t)c->n_l2 * sizeof(double));
+  ALLOC(c->fb.rho, N0 * N1 * sizeof(double));
+  ALLOC(c->fb.ex, N0 * N1 * sizeof(double));
+  ALLOC(c->fb.ey, N0 * N1 * sizeof(double));
+  ALLOC(c->fb.spec1, 2 * nxh * N1 * sizeof(double));
+  ALLOC(c->fb.spec2, 3 * 2 * nxh * N1 * sizeof(double));
+  ALLOC(c->fb.twx, 2 * N0 * sizeof(double));
+  ALLOC(c->fb.twy, 2 * N1 * sizeof(double));
+  ALLOC(c->fb.diag_part, 4 * N1 * sizeof(double));
+  ALLOC(c->fb.diag, 5 * (size_t)(cfg.nbiter + 1) * sizeof(double));
+  ALLOC(c->d_err, sizeof(int));
+#undef ALLOC
+  if (hipHostMalloc((void **)&c->h_err, sizeof(int), hipHostMallocDefault) != hipSuccess)
+    return fail(set_err(nullptr, VLP_E_HIP, "hipHostMalloc failed"));
+  c->fb.nbiter = cfg.nbiter;
+  const std::vector<double> twx = twiddles((int)N0), twy = twiddles((int)N1);
+  if (hipMemcpy(c->fb.twx, twx.data(), twx.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->fb.twy, twy.data(), twy.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(c->fb.diag, 0, 5 * (size_t)(cfg.nbiter + 1) * sizeof(double)) != hipSuccess ||
+      hipMemset(c->d_err, 0, sizeof(int)) != hipSuccess || hipMemset(c->f, 0, c->n_f * c->esz) != hipSuccess ||
+      hipMemset(c->fb.rho, 0, N0 * N1 * sizeof(double)) != hipSuccess ||
+      hipMemset(c->fb.ex, 0, N0 * N1 * sizeof(double)) != hipSuccess ||
+      hipMemset(c->fb.ey, 0, N0 * N1 * sizeof(double)) != hipSuccess)
+    return fail(set_err(nullptr, VLP_E_HIP, "device initialisation failed"));
+  // init(): VMO/Init.cpp:370-372 then VMO/Vlp4d.cpp:30-34
+  const auto t0 = std::chrono::steady_clock::now();
+  if (precision == VLP_FP64) vlp::launch_init_case<double>(c->g, (double *)c->f, c->stream);
+  else vlp::launch_init_case<float>(c->g, (float *)c->f, c->stream);
+  rc = refresh_from_f(c);
+  if (rc) return fail(rc);
+  c->tsec[VLP_T_TOTAL] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  c->tcalls[VLP_T_FIELD]++;
+  c->tcalls[VLP_T_ALLREDUCE]++;
+  c->tcalls[VLP_T_FOURIER]++;
+  c->tcalls[VLP_T_DIAG]++;
+  *out = c;
+  return VLP_OK;
+}
+
+int vlp_destroy(vlp_ctx *c) {
+  if (!c) return VLP_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+  return VLP_OK;
+}
+
+int vlp_step(vlp_ctx *c, int nsteps) {
+  int rc = usable(c);
+  if (rc) return rc;
+  if (nsteps < 0 || c->iter + nsteps > c->cfg.nbiter)
+    return set_err(c, VLP_E_ARG, "step count exceeds nbiter of the configuration");
+  for (int s = 0; s < nsteps; s++) {
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = step_once(c);
+    if (rc) return rc;
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    c->tsec[VLP_T_MAINLOOP] += dt;
+    c->tcalls[VLP_T_MAINLOOP]++;
+    c->tsec[VLP_T_TOTAL] += dt;
+  }
+  return VLP_OK;
+}
+
+int vlp_iter(const vlp_ctx *c) { return c ? c->iter : -1; }
+
+int vlp_get_config(const vlp_ctx *c, vlp_config *cfg) {
+  if (!c || !cfg) return set_err(nullptr, VLP_E_ARG, "null argument");
+  *cfg = c->cfg;
+  return VLP_OK;
+}
+
+int vlp_diags(vlp_ctx *c, double *nrj, double *nrjx, double *nrjy, double *mass, double *l2norm, int n) {
+  int rc = usable(c);
+  if (rc) return rc;
+  if (n < 0 || n > c->iter + 1) return set_err(c, VLP_E_ARG, "n exceeds iter+1");
+  const int nb = c->cfg.nbiter + 1;
+  std::vector<double> h(5 * (size_t)nb);
+  HIP_TRY(c, hipMemcpyAsync(h.data(), c->fb.diag, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  double *outs[5] = {nrj, nrjx, nrjy, mass, l2norm};
+  for (int k = 0; k < 5; k++)
+    if (outs[k]) memcpy(outs[k], h.data() + (size_t)k * nb, (size_t)n * sizeof(double));
+  return VLP_OK;
+}
+
+int vlp_save(vlp_ctx *c, const char *dir) {
+  int rc = usable(c);
+  if (rc) return rc;
+  const int nb = c->cfg.nbiter + 1;
+  std::vector<double> h(5 * (size_t)nb);
+  HIP_TRY(c, hipMemcpyAsync(h.data(), c->fb.diag, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const std::string d = dir ? dir : ".";
+  FILE *f = fopen((d + "/nrj.out").c_str(), "w");  // VMO/Diags.cpp:403-411
+  if (!f) return set_err(c, VLP_E_FILE, "cannot write nrj.out");
+  for (int it = 0; it <= c->cfg.nbiter; ++it)
+    fprintf(f, "%17.13e %17.13e %17.13e %17.13e %17.13e\n", it * c->cfg.dt, h[it], h[nb + it], h[2 * nb + it],
+            h[3 * nb + it]);
+  fclose(f);
+  f = fopen((d + "/l2norm.out").c_str(), "w");  // VMO/Diags.cpp:413-420
+  if (!f) return set_err(c, VLP_E_FILE, "cannot write l2norm.out");
+  for (int it = 0; it <= c->cfg.nbiter; ++it) fprintf(f, "%17.13e %17.13e\n", it * c->cfg.dt, h[4 * nb + it]);
+  fclose(f);
+  return VLP_OK;
+}
+
+int vlp_get_f(vlp_ctx *c, double *host, size_t n) {
+  int rc = usable(c);
+  if (rc) return rc;
+  if (!host || n != c->n_f) return set_err(c, VLP_E_SIZE, "f buffer size mismatch");
+  return download(c, c->f, n, host);
+}
+
+int vlp_set_f(vlp_ctx *c, const double *host, size_t n) {
+  int rc = usable(c);
+  if (rc) return rc;
+  if (!host || n != c->n_f) return set_err(c, VLP_E_SIZE, "f buffer size mismatch");
+  rc = c->precision == VLP_FP64 ? upload_f<double>(c, host) : upload_f<float>(c, host);
+  if (rc) return rc;
+  return refresh_from_f(c);
+}
+
+size_t vlp_buffer_size(const vlp_ctx *c, int which) {
+  if (!c) return 0;
+  const size_t n2 = (size_t)c->cfg.nxmax[0] * c->cfg.nxmax[1];
+  switch (which) {
+    case VLP_BUF_F: return c->n_f;
+    case VLP_BUF_CXY: return c->n_cxy;
+    case VLP_BUF_C4: return c->n_c4;
+    case VLP_BUF_RHO:
+    case VLP_BUF_EX:
+    case VLP_BUF_EY: return n2;
+    default: return 0;
+  }
+}
+
+int vlp_get_buffer(vlp_ctx *c, int which, double *host, size_t n) {
+  int rc = usable(c);
+  if (rc) return rc;
+  if (!host || n != vlp_buffer_size(c, which) || n == 0) return set_err(c, VLP_E_SIZE, "buffer size mismatch");
+  switch (which) {
+    case VLP_BUF_F: return download(c, c->f, n, host);
+    case VLP_BUF_CXY: return download(c, c->cxy, n, host);
+    case VLP_BUF_C4: return download(c, c->c4, n, host);
+    default: break;
+  }
+  const double *src = which == VLP_BUF_RHO ? c->fb.rho : which == VLP_BUF_EX ? c->fb.ex : c->fb.ey;
+  HIP_TRY(c, hipMemcpyAsync(host, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return VLP_OK;
+}
+
+int vlp_spline_xy(vlp_ctx *c) {
+  int rc = usable(c);
+  if (rc) return rc;
+  if (c->precision == VLP_FP64) run_spline_xy<double>(c); else run_spline_xy<float>(c);
+  rc = check_launch(c, "spline_xy");
+  if (rc) return rc;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return VLP_OK;
+}
+
+int vlp_advect2d_poisson(vlp_ctx *c) {
+  int rc = usable(c);
+  if (rc) return rc;
+  if (c->iter >= c->cfg.nbiter) return set_err(c, VLP_E_ARG, "iteration count would exceed nbiter");
+  if (c->precision == VLP_FP64) run_advect2d<double>(c); else run_advect2d<float>(c);
+  rc = field(c, c->iter + 1, false, false);
+  if (rc) return rc;
+  return sync_and_check(c);
+}
+
+int vlp_spline_vxvy(vlp_ctx *c) {
+  int rc = usable(c);
+  if (rc) return rc;
+  if (c->precision == VLP_FP64) run_spline_vxvy<double>(c); else run_spline_vxvy<float>(c);
+  rc = check_launch(c, "spline_vxvy");
+  if (rc) return rc;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return VLP_OK;
+}
+
+int vlp_advect4d_poisson(vlp_ctx *c) {
+  int rc = usable(c);
+  if (rc) return rc;
+  if (c->iter >= c->cfg.nbiter) return set_err(c, VLP_E_ARG, "iteration count would exceed nbiter");
+  if (c->precision == VLP_FP64) run_advect4d<double>(c); else run_advect4d<float>(c);
+  rc = field(c, c->iter + 1, true, true);
+  if (rc) return rc;
+  std::swap(c->f, c->fnext);
+  c->iter++;
+  return sync_and_check(c);
+}
+
+int vlp_timer(vlp_ctx *c, int which, double *seconds, int *calls) {
+  if (!c || which < 0 || which >= VLP_T_COUNT) return set_err(c, VLP_E_ARG, "bad timer index");
+  if (seconds) *seconds = c->tsec[which];
+  if (calls) *calls = c->tcalls[which];
+  return VLP_OK;
+}
+
+const char *vlp_timer_label(int which) { return (which >= 0 && which < VLP_T_COUNT) ? kLabels[which] : ""; }
+
+const char *vlp_last_error(const vlp_ctx *c) { return c ? c->err.c_str() : g_last_error.c_str(); }
+
+}  // extern "C"

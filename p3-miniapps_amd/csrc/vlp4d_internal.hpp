@@ -1,0 +1,65 @@
+// vlp4d_internal.hpp -- shared declarations of the HIP library (not part of the C-ABI).
+//
+// Device data layout (all x fastest, like the reference's layout_left views):
+//   F    : interior f, N0*N1*N2*N3                         (positions 0..N-1 per dim)
+//   CXY  : (N0+4)*(N1+4)*N2*N3, x,y on -2..N+1, v interior  (spline along x and y)
+//   TV   : (N0+4)*(N1+4)*(N2+4)*N3, also holds the x-pass output (N0+4)*N1*N2*N3
+//   C4   : (N0+4)*(N1+4)*(N2+4)*(N3+4), all dims on -2..N+1
+// The reference's (N+6)^4 halo arrays are never materialised: the periodic
+// copies and alpha-sums of its halo exchange are computed inside the spline
+// kernels from the interior line they extend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace vlp {
+
+// Everything a kernel needs from the configuration (one rank).
+struct Geom {
+  int n[4];        // points per dimension
+  double minPhy[4];
+  double maxPhy[4];
+  double dx[4];
+  double inv_dx[4];  // 1./dx, VMO/Advection.hpp:319,430
+  int idcase;
+};
+
+// Launchers (kernels.hip).  T is the storage type (double or float).
+template <typename T> void launch_init_case(const Geom &g, T *f, hipStream_t s);
+template <typename T> void launch_spline_x(const Geom &g, const T *f, T *tx, hipStream_t s);
+// Strided-line pass: lines of N interior values along a dimension with
+// stride in_ls (input) / out_ls (output); "inner" index i in [0,X) is
+// contiguous (stride 1) in both arrays, "outer" o in [0,A) has strides
+// in_os / out_os.  Output holds positions -2..N+1.
+template <typename T>
+void launch_spline_line(const T *in, T *out, int N, long X, long A, long in_ls, long in_os, long out_ls, long out_os,
+                        hipStream_t s);
+template <typename T>
+void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, double *rho_part, int *err, hipStream_t s);
+template <typename T>
+void launch_advect4d(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, T *fout,
+                     double *rho_part, double *l2_part, int *err, hipStream_t s);
+template <typename T>
+void launch_moments(const Geom &g, const T *f, double *rho_part, double *l2_part, hipStream_t s);
+
+// Number of l2 partial sums written by launch_advect4d / launch_moments.
+long l2_partials(const Geom &g);
+
+// Field solve (FP64): rho from the per-vy partial sums, Poisson by DFT with
+// the reference's FFT conventions, field diagnostics; optional L2 norm.
+struct FieldBufs {
+  double *rho;     // N0*N1 : rho, then phi (Poisson cases)
+  double *ex, *ey; // N0*N1
+  double *spec1;   // (N0/2+1)*N1 complex: r2c along x
+  double *spec2;   // 3 * (N0/2+1)*N1 complex: inverse along y (phi, ex, ey)
+  double *twx, *twy;  // exp(-2 pi i m/N) tables, complex
+  double *diag_part;  // 4*N1
+  double *diag;       // 5*(nbiter+1): nrj, nrjx, nrjy, mass, l2
+  int nbiter;
+};
+void launch_field(const Geom &g, const FieldBufs &b, const double *rho_part, const double *l2_part, long n_l2,
+                  int iter, bool record_diag, hipStream_t s);
+
+}  // namespace vlp

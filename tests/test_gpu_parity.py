@@ -1,0 +1,120 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+Each phase of onetimestep (VMO/Timestep.hpp:34-97) is fed the same input on
+both sides and compared in relative max-norm (max|gpu-oracle| / max|oracle|).
+The bar is the north star's 1e-12 for FP64; phase-level checks use 1e-13
+(single applications; differences are FMA contraction and summation order).
+The grids are deliberately non-cubic so that a swapped dimension or stride
+cannot pass.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import relmax, sld_box
+
+pytestmark = pytest.mark.gpu
+
+# (Nx, Ny, Nvx, Nvy) non-uniform; CFL-valid dt (dt*V <= dx).
+SHAPES = [(16, 12, 20, 14), (24, 20, 16, 18)]
+
+
+def _configs(vlp, orc, n, idcase=10, dt=0.05, nbiter=8, vmax=6.0, lx=4 * math.pi):
+    args = sld_box(n, vmax=vmax, dt=dt, nbiter=nbiter, idcase=idcase, lx=lx)
+    return vlp.VlpConfig.make(*args), orc.make_config(*args)
+
+
+def _cxy_view(orc, h):
+    # x,y on -2..N+1, vx,vy interior
+    return h[3:-3, 3:-3, 1:-1, 1:-1]
+
+
+@pytest.mark.parametrize("n", SHAPES)
+def test_init_matches_oracle(vlp, orc, n):
+    gc, oc = _configs(vlp, orc, n)
+    with vlp.Vlp4d(gc) as sim:
+        f = sim.get_f()
+    ref = orc.interior_of(orc.init_case(oc))
+    assert relmax(f, ref) < 1e-14
+
+
+@pytest.mark.parametrize("n", SHAPES)
+def test_phases_match_oracle(vlp, orc, n):
+    gc, oc = _configs(vlp, orc, n)
+    rng = np.random.default_rng(1234)
+    h = orc.init_case(oc)
+    # perturb so that no symmetry hides an index error
+    orc.interior_of(h)[...] *= 1.0 + 0.1 * rng.standard_normal(orc.interior_shape(oc))
+    f0 = np.ascontiguousarray(orc.interior_of(h))
+    with vlp.Vlp4d(gc) as sim:
+        sim.set_f(f0)
+        # exchangeHalo + computeCoeff_xy
+        orc.exchange_halo(oc, h)
+        orc.spline_xy(oc, h)
+        fnp1 = h.copy()
+        sim.spline_xy()
+        cxy = sim.buffer(vlp.BUF_CXY)
+        assert relmax(cxy, _cxy_view(orc, h)) < 1e-13
+        # advect_2D_xy(dt/2) + field_rho/reduce/poisson
+        assert orc.advect_2d(oc, h, 0.5 * oc.dt) == 0
+        rho = orc.field_rho(oc, h)
+        phi, ex, ey = orc.field_poisson(oc, rho)
+        sim.advect2d_poisson()
+        assert relmax(sim.buffer(vlp.BUF_EX), ex) < 1e-12
+        assert relmax(sim.buffer(vlp.BUF_EY), ey) < 1e-12
+        assert relmax(sim.buffer(vlp.BUF_RHO), phi) < 1e-12
+        # computeCoeff_vxvy
+        orc.spline_vxvy(oc, fnp1)
+        sim.spline_vxvy()
+        assert relmax(sim.buffer(vlp.BUF_C4), orc.ext_view(fnp1)) < 1e-13
+        # advect_4D + field + diags (oracle uses its own E)
+        assert orc.advect_4d(oc, ex, ey, fnp1, oc.dt) == 0
+        sim.advect4d_poisson()
+        assert relmax(sim.get_f(), orc.interior_of(fnp1)) < 1e-12
+        rho2 = orc.field_rho(oc, fnp1)
+        phi2, ex2, ey2 = orc.field_poisson(oc, rho2)
+        d = orc.diags_field(oc, phi2, ex2, ey2)
+        g = sim.diags()
+        assert abs(g["nrj"][1] - d[0]) <= 1e-12 * abs(d[0])
+        assert abs(g["nrjx"][1] - d[1]) <= 1e-12 * abs(d[1])
+        assert abs(g["nrjy"][1] - d[2]) <= 1e-12 * abs(d[2])
+        noise = 2.2e-16 * np.sum(np.abs(phi2)) * oc.dx[0] * oc.dx[1]
+        assert abs(g["mass"][1]) < 16 * noise and abs(d[3]) < 16 * noise
+        l2 = orc.l2norm(oc, fnp1)
+        assert abs(g["l2norm"][1] - l2) <= 1e-12 * l2
+
+
+@pytest.mark.parametrize("idcase,n,steps,vmax,lx", [
+    (10, (16, 16, 16, 16), 20, 6.0, 4 * math.pi),
+    (20, (20, 12, 24, 16), 12, 9.0, 10 * math.pi),
+])
+def test_multistep_matches_oracle(vlp, orc, idcase, n, steps, vmax, lx):
+    dt = 0.5 * (lx / n[0]) / vmax
+    gc, oc = _configs(vlp, orc, n, idcase=idcase, dt=dt, nbiter=steps, vmax=vmax, lx=lx)
+    st = orc.State(oc)
+    st.step(steps)
+    ref_f = st.f_interior()
+    ref_d = st.diags()
+    with vlp.Vlp4d(gc) as sim:
+        sim.step(steps)
+        f = sim.get_f()
+        d = sim.diags()
+    assert relmax(f, ref_f) < 1e-12
+    for k in ("nrj", "nrjx", "nrjy", "l2norm"):
+        assert relmax(d[k], ref_d[k]) < 1e-12, k
+    # mass = sum(phi) dx dy is round-off by construction (phi has no k=0
+    # mode, VMO/Efield.cpp:38): both sides are noise of size eps*sum|phi|dxdy.
+    phi = st.field(0)
+    noise = 2.2e-16 * np.sum(np.abs(phi)) * oc.dx[0] * oc.dx[1]
+    assert np.max(np.abs(d["mass"])) < 16 * noise
+    assert np.max(np.abs(ref_d["mass"])) < 16 * noise
+
+
+def test_cfl_violation_reports_error(vlp):
+    # dt*V/dx = 3 > 1: the reference prints "Time step is too large" and exits.
+    args = sld_box(16, vmax=6.0, dt=3 * (4 * math.pi / 16) / 6.0, nbiter=2)
+    with vlp.Vlp4d(vlp.VlpConfig.make(*args)) as sim:
+        with pytest.raises(vlp.Vlp4dError) as ei:
+            sim.step(1)
+        assert ei.value.code == vlp.E_CFL
