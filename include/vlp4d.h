@@ -144,6 +144,12 @@ int vlp_advect4d_poisson(vlp_ctx *ctx);
 int vlp_timer(vlp_ctx *ctx, int which, double *seconds, int *calls);
 const char *vlp_timer_label(int which);
 
+/* Which kernel variants this context's geometry selects (bit 0: multi-lane
+ * scan for the x spline, bit 1: register-line kernels for the y/vx/vy
+ * splines, bit 2: vx-blocked 4-D advection).  No reference counterpart:
+ * introspection for tests and reports. */
+int vlp_kernel_variants(const vlp_ctx *ctx);
+
 /* Message of the last error on this context (or the last global error when
  * ctx is NULL). */
 const char *vlp_last_error(const vlp_ctx *ctx);

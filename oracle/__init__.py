@@ -65,6 +65,7 @@ def load():
     lib.vo_spline_vxvy.argtypes = [C, dp]
     lib.vo_advect_2d.argtypes = [C, dp, dp, ctypes.c_double]
     lib.vo_advect_4d.argtypes = [C, dp, dp, dp, dp, ctypes.c_double]
+    lib.vo_feet.argtypes = [C, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
     lib.vo_field_rho.argtypes = [C, dp, dp]
     lib.vo_field_poisson.argtypes = [C, dp, dp, dp]
     lib.vo_diags_field.argtypes = [C, dp, dp, dp, dp]
@@ -172,6 +173,12 @@ def advect_2d(c, h, dt):
 def advect_4d(c, ex, ey, h, dt):
     tmp = np.empty_like(h)
     return load().vo_advect_4d(ctypes.byref(c), _p(ex), _p(ey), _p(h), _p(tmp), dt)
+
+
+def feet(c, ex, ey, ix, iy, ivx, ivy, dt):
+    out = np.zeros(4)
+    load().vo_feet(ctypes.byref(c), _p(ex), _p(ey), ix, iy, ivx, ivy, dt, _p(out))
+    return out
 
 
 def field_rho(c, h):

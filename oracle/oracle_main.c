@@ -57,6 +57,7 @@ int main(int argc, char **argv) {
   int rc = vo_parse_deck(file, &cfg);
   if (rc == VO_E_FILE) {
     printf("import: error file not found\n");
+    fflush(stdout);
     abort();
   }
   print_config(&cfg);
@@ -65,6 +66,7 @@ int main(int argc, char **argv) {
   rc = vo_create(&cfg, &st);
   if (rc == VO_E_CASE) {
     printf("Unknown test case !\n");
+    fflush(stdout);
     abort();
   }
   if (rc) {

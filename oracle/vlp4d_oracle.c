@@ -573,6 +573,23 @@ static inline double interp_4d(const vo_config *c, const double *tmp_fn, const d
   return ftmp1;
 }
 
+/* Foot of one interior cell (computeFeet with advect_4D's constants), exported
+ * for tests. */
+void vo_feet(const vo_config *c, const double *ex, const double *ey, int ix, int iy, int ivx, int ivy, double dt,
+             double xstar[4]) {
+  double rxmin[4], inv_dx[4], rxwidth[4], dx[4];
+  int xmax[4];
+  for (int j = 0; j < 4; j++) {
+    xmax[j] = c->nxmax[j];
+    rxmin[j] = c->minPhy[j];
+    rxwidth[j] = c->maxPhy[j] - c->minPhy[j];
+    inv_dx[j] = 1. / c->dx[j];
+    dx[j] = c->dx[j];
+  }
+  const int pos[4] = {ix, iy, ivx, ivy};
+  compute_feet(xstar, ex, ey, rxmin, rxwidth, dx, inv_dx, xmax, pos, dt);
+}
+
 /* fn holds C4 on entry; on exit its interior is f^{n+1}; tmp_fn receives a copy
  * of C4 (the reference's argument order, VMO/Timestep.hpp:74). */
 int vo_advect_4d(const vo_config *c, const double *ex, const double *ey, double *fn, double *tmp_fn, double dt) {

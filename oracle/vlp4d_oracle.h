@@ -75,6 +75,8 @@ void vo_spline_vxvy(const vo_config *cfg, double *fn);       /* VMO/Spline.hpp:4
 int vo_advect_2d(const vo_config *cfg, double *fn, double *tmp, double dt);           /* VMO/Advection.hpp:299-406 */
 int vo_advect_4d(const vo_config *cfg, const double *ex, const double *ey,
                  double *fn, double *tmp_fn, double dt);                               /* VMO/Advection.hpp:409-498 */
+void vo_feet(const vo_config *cfg, const double *ex, const double *ey, int ix, int iy, int ivx, int ivy,
+             double dt, double xstar[4]);                                                /* VMO/Advection.hpp:166-218 */
 void vo_field_rho(const vo_config *cfg, const double *fn, double *rho);               /* VMO/Field.cpp:5-55 */
 void vo_field_poisson(const vo_config *cfg, double *rho, double *ex, double *ey);     /* VMO/Field.cpp:71-137 */
 void vo_diags_field(const vo_config *cfg, const double *rho, const double *ex,

@@ -33,6 +33,7 @@ EXPORTS = [
     "vlp_iter", "vlp_get_config", "vlp_diags", "vlp_save", "vlp_get_f", "vlp_set_f",
     "vlp_buffer_size", "vlp_get_buffer", "vlp_spline_xy", "vlp_advect2d_poisson",
     "vlp_spline_vxvy", "vlp_advect4d_poisson", "vlp_timer", "vlp_timer_label", "vlp_last_error",
+    "vlp_kernel_variants",
 ]
 
 
@@ -110,6 +111,7 @@ def load(path=LIB_PATH):
     lib.vlp_timer_label.restype = ctypes.c_char_p
     lib.vlp_last_error.argtypes = [vp]
     lib.vlp_last_error.restype = ctypes.c_char_p
+    lib.vlp_kernel_variants.argtypes = [vp]
     _lib = lib
     return lib
 
@@ -220,6 +222,10 @@ class Vlp4d:
 
     def save(self, directory="."):
         self._check(self.lib.vlp_save(self.h, os.fsencode(directory)))
+
+    def kernel_variants(self):
+        """bit 0: scan x-spline, bit 1: register-line y/vx/vy splines, bit 2: blocked 4-D advection."""
+        return self.lib.vlp_kernel_variants(self.h)
 
     def timers(self):
         out = {}

@@ -1,0 +1,453 @@
+// advect.hip -- advection, moments and initial-condition kernels (gfx950).
+//
+// Storage type T is double (plain path) or float (FP32 storage); feet and the
+// density/L2 reductions are always computed in double.
+#include "device_common.hpp"
+
+#include <cstdlib>
+
+namespace vlp {
+
+// ---------------------------------------------------------------------------
+// 2-D half-step advection fused with the density reduction:
+// VMO/Advection.hpp:299-406 (spline interp_2D :126-163) and VMO/Field.cpp:37-53.
+// One lane per (x, y, vy), looping over vx: rho_part[vy][y][x] = sum_vx f_half,
+// the same summation order as field_rho's inner loop.  f_half is not stored.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_advect2d_rho(const T *__restrict__ cxy, Geom g, double dt,
+                                                      double *__restrict__ rho_part, int *__restrict__ err) {
+  using A = T;
+  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)N0 * N1 * N3) return;
+  const int ix = (int)(t % N0);
+  const long r = t / N0;
+  const int iy = (int)(r % N1);
+  const int ivy = (int)(r / N1);
+  const long E0 = N0 + 4, E1 = N1 + 4;
+  const double y = g.minPhy[1] + iy * g.dx[1];
+  const double vy = g.minPhy[3] + ivy * g.dx[3];
+  const double depy = dt * vy;
+  const double fy = ((y - depy) - g.minPhy[1]) * g.inv_dx[1];
+  const int jy0 = (int)floor(fy);
+  A etay[4];
+  bspline_eta<A>((A)(fy - (double)jy0), etay);
+  const bool ybad = (jy0 < -1 || jy0 > N1 - 1);
+  const double x = g.minPhy[0] + ix * g.dx[0];
+  double sum = 0.;
+  int bad = 0;
+  for (int ivx = 0; ivx < N2; ivx++) {
+    const double vx = g.minPhy[2] + ivx * g.dx[2];
+    const double depx = dt * vx;
+    const double fx = ((x - depx) - g.minPhy[0]) * g.inv_dx[0];
+    const int jx0 = (int)floor(fx);
+    A etax[4];
+    bspline_eta<A>((A)(fx - (double)jx0), etax);
+    A val = 0;
+    if (ybad || jx0 < -1 || jx0 > N0 - 1) {
+      bad = 1;  // interp_2D returns 1 and leaves 0 (VMO/Advection.hpp:148-151)
+    } else {
+      // position jx0-1 sits at storage index jx0+1
+      const T *p = cxy + ((long)ivx + (long)N2 * ivy) * E0 * E1 + (jx0 + 1) + E0 * (jy0 + 1);
+      for (int jy = 0; jy < 4; jy++) {
+        A s = 0;
+        for (int jx = 0; jx < 4; jx++) s += etax[jx] * (A)p[jx + E0 * jy];
+        val += etay[jy] * s;
+      }
+    }
+    sum += (double)val;
+  }
+  rho_part[ix + (long)N0 * (iy + (long)N1 * ivy)] = sum;
+  if (bad) atomicOr(err, 1);
+}
+
+template <typename T>
+void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, double *rho_part, int *err, hipStream_t s) {
+  const long nthr = (long)g.n[0] * g.n[1] * g.n[3];
+  hipLaunchKernelGGL(k_advect2d_rho<T>, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, cxy, g, dt_half,
+                     rho_part, err);
+}
+
+// ---------------------------------------------------------------------------
+// 4-D advection fused with rho and the L2 norm:
+// computeFeet VMO/Advection.hpp:166-218, interp_4D :221-296, advect_4D
+// :409-498, field_rho VMO/Field.cpp:37-53, computeL2norm VMO/Diags.cpp:50-76.
+// ---------------------------------------------------------------------------
+struct FeetConsts {
+  double rxwidth[4];
+  double locmin[4], locmax[4];  // VMO/Advection.hpp:432-433
+};
+
+#pragma clang fp contract(off)
+static FeetConsts feet_consts(const Geom &g) {
+  FeetConsts f;
+  for (int j = 0; j < 4; j++) {
+    f.rxwidth[j] = g.maxPhy[j] - g.minPhy[j];
+    f.locmin[j] = g.minPhy[j] + 0 * g.dx[j] - g.dx[j];
+    f.locmax[j] = g.minPhy[j] + (g.n[j] - 1) * g.dx[j] + g.dx[j];
+  }
+  return f;
+}
+#pragma clang fp contract(on)
+
+// Characteristic foot of cell (x, y, vx, vy), computeFeet of
+// VMO/Advection.hpp:166-218: half-step position, 3x3 Lagrange interpolation
+// of E with periodic wrap, full-step foot, velocity clamped to the box.
+// Returns false when the foot leaves [min-dx, min+N dx] in any dimension
+// (NaN counts as outside).
+__device__ __forceinline__ bool compute_foot(const Geom &g, const FeetConsts &fc, const double *__restrict__ ex,
+                                             const double *__restrict__ ey, double x, double y, double vx, double vy,
+                                             double dt, double xs[4]) {
+  const int N0 = g.n[0], N1 = g.n[1];
+  double xt0 = x - 0.5 * dt * vx;
+  double xt1 = y - 0.5 * dt * vy;
+  xt0 = g.inv_dx[0] * (xt0 - g.minPhy[0]);
+  xt1 = g.inv_dx[1] * (xt1 - g.minPhy[1]);
+  const int ip0 = (int)(round(xt0) - 1);
+  const int ip1 = (int)(round(xt1) - 1);
+  const double px = xt0 - ip0, py = xt1 - ip1;
+  const double cx[3] = {.5 * (px - 1.) * (px - 2.), -1. * (px) * (px - 2.), .5 * (px) * (px - 1.)};
+  const double cy[3] = {.5 * (py - 1.) * (py - 2.), -1. * (py) * (py - 2.), .5 * (py) * (py - 1.)};
+  double ftmp1 = 0., ftmp2 = 0.;
+  for (int k1 = 0; k1 <= 2; k1++) {
+    const int jy = ((ip1 + k1) % N1 + N1) % N1;  // (N + i) % N, non-negative for any i
+    for (int k0 = 0; k0 <= 2; k0++) {
+      const int jx = ((ip0 + k0) % N0 + N0) % N0;
+      const double cc = cx[k0] * cy[k1];
+      ftmp1 += cc * ex[jx + (long)N0 * jy];
+      ftmp2 += cc * ey[jx + (long)N0 * jy];
+    }
+  }
+  const double xt2 = vx - 0.5 * dt * ftmp1;
+  const double xt3 = vy - 0.5 * dt * ftmp2;
+  xs[0] = x - dt * xt2;
+  xs[1] = y - dt * xt3;
+  {
+    const double a = vx - dt * ftmp1, hi = g.minPhy[2] + fc.rxwidth[2], lo = g.minPhy[2];
+    const double m = (hi < a) ? hi : a;
+    xs[2] = (m < lo) ? lo : m;
+  }
+  {
+    const double a = vy - dt * ftmp2, hi = g.minPhy[3] + fc.rxwidth[3], lo = g.minPhy[3];
+    const double m = (hi < a) ? hi : a;
+    xs[3] = (m < lo) ? lo : m;
+  }
+  bool ok = true;
+  for (int j = 0; j < 4; j++) ok = ok && (xs[j] >= fc.locmin[j] && xs[j] <= fc.locmax[j]);
+  return ok;
+}
+
+// Direct 4^4 gather of interp_4D on the C4 layout (positions -3..N+2, stride
+// F_d = N_d + 6).  A valid foot has ip in [-2, N], so positions ip-1..ip+2
+// stay inside -3..N+2; any reading of -3 or N+2 carries weight 0 (or ~1e-16
+// at the lower edge), where the stored value is 0.
+template <typename T, typename A>
+__device__ __forceinline__ A gather4d(const T *__restrict__ c4, long S1, long S2, long S3, const int ip[4],
+                                      const A eta[4][4]) {
+  const T *base = c4 + (ip[0] + 2) + S1 * (ip[1] + 2) + S2 * (ip[2] + 2) + S3 * (long)(ip[3] + 2);
+  A ftmp1 = 0;
+  for (int k3 = 0; k3 < 4; k3++)
+    for (int k2 = 0; k2 < 4; k2++) {
+      A ftmp2 = 0;
+      const T *p23 = base + k3 * S3 + k2 * S2;
+      for (int k1 = 0; k1 < 4; k1++) {
+        const T *p = p23 + k1 * S1;
+        A ftmp3 = 0;
+        for (int k0 = 0; k0 < 4; k0++) ftmp3 += eta[0][k0] * (A)p[k0];
+        ftmp2 += ftmp3 * eta[1][k1];
+      }
+      ftmp1 += ftmp2 * eta[2][k2] * eta[3][k3];
+    }
+  return ftmp1;
+}
+
+// Lane per (x, y, vy); the lane walks vx in blocks of L cells.  In a block the
+// L feet normally share the y- and vy-stencil, their vx-stencils are shifted
+// by exactly one per cell and their x-stencils differ by at most one: then the
+// block reads the union of its stencils once -- rows of 5 x-values over L+3
+// vx-slices -- and every row feeds up to 4 cells (x offset folded into a
+// zero-padded 5-weight vector).  Blocks that do not have this structure, and
+// the tail block, fall back to the direct gather.
+template <typename T, int L>
+__global__ __launch_bounds__(256) void k_advect4d_blk(const T *__restrict__ c4, const double *__restrict__ ex,
+                                                      const double *__restrict__ ey, Geom g, FeetConsts fc, double dt,
+                                                      T *__restrict__ fout, double *__restrict__ rho_part,
+                                                      double *__restrict__ l2_part, int *__restrict__ err) {
+  using A = T;
+  __shared__ double sh[8];
+  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = t < (long)N0 * N1 * N3;
+  double l2 = 0.;
+  int bad = 0;
+  if (active) {
+    const int ix = (int)(t % N0);
+    const long r = t / N0;
+    const int iy = (int)(r % N1);
+    const int ivy = (int)(r / N1);
+    const long S1 = N0 + 6, S2 = S1 * (N1 + 6), S3 = S2 * (N2 + 6);
+    const double x = g.minPhy[0] + ix * g.dx[0];
+    const double y = g.minPhy[1] + iy * g.dx[1];
+    const double vy = g.minPhy[3] + ivy * g.dx[3];
+    T *orow = fout + ix + (long)N0 * (iy + (long)N1 * (long)N2 * ivy);
+    const long ostride = (long)N0 * N1;
+    double rho = 0.;
+    for (int v0 = 0; v0 < N2; v0 += L) {
+      int ip[L][4];
+      A wx[L];         // fractional x position; x weights built after the alignment test
+      A eta[L][3][4];  // weights along y, vx, vy
+      bool ok[L];
+      bool fast = true;
+#pragma unroll
+      for (int c = 0; c < L; c++) {
+        const int ivx = v0 + c;
+        ok[c] = false;
+        if (ivx < N2) {
+          const double vx = g.minPhy[2] + ivx * g.dx[2];
+          double xs[4];
+          ok[c] = compute_foot(g, fc, ex, ey, x, y, vx, vy, dt, xs);
+          if (ok[c]) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+              const double fj = g.inv_dx[j] * (xs[j] - g.minPhy[j]);
+              ip[c][j] = (int)floor(fj);
+              const A w = (A)(fj - ip[c][j]);
+              if (j == 0) wx[c] = w;
+              else bspline_eta<A>(w, eta[c][j - 1]);
+            }
+          }
+        }
+        fast = fast && ok[c];
+      }
+      int bx0 = 0;
+      if (fast) {
+        bx0 = ip[0][0];
+#pragma unroll
+        for (int c = 1; c < L; c++) bx0 = ip[c][0] < bx0 ? ip[c][0] : bx0;
+#pragma unroll
+        for (int c = 0; c < L; c++)
+          fast = fast && (ip[c][0] - bx0 <= 1) && (ip[c][1] == ip[0][1]) && (ip[c][3] == ip[0][3]) &&
+                 (ip[c][2] == ip[0][2] + c);
+      }
+      A val[L];
+      if (fast) {
+        A e0[L][5];
+#pragma unroll
+        for (int c = 0; c < L; c++) {
+          A ex4[4];
+          bspline_eta<A>(wx[c], ex4);
+          const bool sh1 = ip[c][0] != bx0;
+#pragma unroll
+          for (int i = 0; i < 5; i++) {
+            const A lo = (i < 4) ? ex4[i] : (A)0;
+            const A hi = (i > 0) ? ex4[i - 1] : (A)0;
+            e0[c][i] = sh1 ? hi : lo;
+          }
+          val[c] = 0;
+        }
+        const T *base = c4 + (bx0 + 2) + S1 * (ip[0][1] + 2) + S2 * (ip[0][2] + 2) + S3 * (long)(ip[0][3] + 2);
+#pragma unroll
+        for (int j = 0; j < L + 3; j++) {
+#pragma unroll
+          for (int k3 = 0; k3 < 4; k3++) {
+            A tk[L];
+#pragma unroll
+            for (int c = 0; c < L; c++) tk[c] = 0;
+#pragma unroll
+            for (int k1 = 0; k1 < 4; k1++) {
+              const T *rowp = base + j * S2 + k3 * S3 + k1 * S1;
+              A rv[5];
+#pragma unroll
+              for (int i = 0; i < 5; i++) rv[i] = (A)rowp[i];
+#pragma unroll
+              for (int c = 0; c < L; c++) {
+                const int k2 = j - c;
+                if (k2 >= 0 && k2 < 4) {
+                  A s = 0;
+#pragma unroll
+                  for (int i = 0; i < 5; i++) s += e0[c][i] * rv[i];
+                  tk[c] += s * eta[c][0][k1];
+                }
+              }
+            }
+#pragma unroll
+            for (int c = 0; c < L; c++) {
+              const int k2 = j - c;
+              if (k2 >= 0 && k2 < 4) val[c] += tk[c] * eta[c][1][k2] * eta[c][2][k3];
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < L; c++) {
+          val[c] = 0;
+          if (ok[c]) {
+            A e4[4][4];
+            bspline_eta<A>(wx[c], e4[0]);
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+#pragma unroll
+              for (int k = 0; k < 4; k++) e4[j + 1][k] = eta[c][j][k];
+            val[c] = gather4d<T, A>(c4, S1, S2, S3, ip[c], e4);
+          } else if (v0 + c < N2) {
+            bad = 1;
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < L; c++) {
+        if (v0 + c < N2) {
+          orow[(long)(v0 + c) * ostride] = (T)val[c];
+          rho += (double)val[c];
+          l2 += (double)val[c] * (double)val[c];
+        }
+      }
+    }
+    rho_part[ix + (long)N0 * (iy + (long)N1 * ivy)] = rho;
+  }
+  const double tot = block_sum(l2, sh);
+  if (threadIdx.x == 0) l2_part[blockIdx.x] = tot;
+  if (bad) atomicOr(err, 2);
+}
+
+long l2_partials(const Geom &g) { return ((long)g.n[0] * g.n[1] * g.n[3] + 255) / 256; }
+
+// Cells per lane block in the 4-D advection (register blocking along vx).
+// Tuning knob: VLP4D_ADVECT_L=2|4 overrides the default.
+static int advect_block(int bits) {
+  static int env = -1;
+  if (env < 0) {
+    const char *e = getenv("VLP4D_ADVECT_L");
+    env = e ? atoi(e) : 0;
+  }
+  if (env == 2 || env == 4) return env;
+  return bits == 64 ? 4 : 4;
+}
+
+template <typename T>
+void launch_advect4d(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, T *fout,
+                     double *rho_part, double *l2_part, int *err, hipStream_t s) {
+  const dim3 grid((unsigned)l2_partials(g));
+  if (advect_block(8 * (int)sizeof(T)) == 2)
+    hipLaunchKernelGGL((k_advect4d_blk<T, 2>), grid, dim3(256), 0, s, c4, ex, ey, g, feet_consts(g), dt, fout,
+                       rho_part, l2_part, err);
+  else
+    hipLaunchKernelGGL((k_advect4d_blk<T, 4>), grid, dim3(256), 0, s, c4, ex, ey, g, feet_consts(g), dt, fout,
+                       rho_part, l2_part, err);
+}
+
+// moments of a given f: rho partials (sum over vx) and L2 partials
+template <typename T>
+__global__ __launch_bounds__(256) void k_moments(const T *__restrict__ f, Geom g, double *__restrict__ rho_part,
+                                                 double *__restrict__ l2_part) {
+  __shared__ double sh[8];
+  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double l2 = 0.;
+  if (t < (long)N0 * N1 * N3) {
+    const int ix = (int)(t % N0);
+    const long r = t / N0;
+    const int iy = (int)(r % N1);
+    const int ivy = (int)(r / N1);
+    double rho = 0.;
+    for (int ivx = 0; ivx < N2; ivx++) {
+      const double v = (double)f[ix + (long)N0 * (iy + (long)N1 * (ivx + (long)N2 * ivy))];
+      rho += v;
+      l2 += v * v;
+    }
+    rho_part[ix + (long)N0 * (iy + (long)N1 * ivy)] = rho;
+  }
+  const double tot = block_sum(l2, sh);
+  if (threadIdx.x == 0) l2_part[blockIdx.x] = tot;
+}
+
+template <typename T> void launch_moments(const Geom &g, const T *f, double *rho_part, double *l2_part, hipStream_t s) {
+  hipLaunchKernelGGL(k_moments<T>, dim3((unsigned)l2_partials(g)), dim3(256), 0, s, f, g, rho_part, l2_part);
+}
+
+// ---------------------------------------------------------------------------
+// Initial conditions, VMO/Init.cpp:126-277 (f must be zero on entry).
+// ---------------------------------------------------------------------------
+template <typename T> __global__ __launch_bounds__(256) void k_init_case(T *__restrict__ f, Geom g) {
+  const long n = (long)g.n[0] * g.n[1] * g.n[2] * g.n[3];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int ix = (int)(t % g.n[0]);
+  long r = t / g.n[0];
+  const int iy = (int)(r % g.n[1]);
+  r /= g.n[1];
+  const int ivx = (int)(r % g.n[2]);
+  const int ivy = (int)(r / g.n[2]);
+  double val = 0.;
+  switch (g.idcase) {
+    case 2:
+    case 5: {  // CYL02 / CYL05, VMO/Init.cpp:152-224
+      const double PI = M_PI, AMPLI = 4, PERIOD = 0.5 * PI;
+      const double cc = 0.50 * (6. / 16.), rc = 0.50 * (4. / 16.);
+      double xx, vv;
+      if (g.idcase == 2) {
+        xx = g.minPhy[0] + ix * g.dx[0];
+        vv = g.minPhy[2] + ivx * g.dx[2];
+      } else {
+        // CYL05 loops stop one short in ivx, iy, ix (VMO/Init.cpp:196-199)
+        if (ivx >= g.n[2] - 1 || iy >= g.n[1] - 1 || ix >= g.n[0] - 1) return;
+        xx = g.minPhy[1] + iy * g.dx[1];
+        vv = g.minPhy[3] + ivy * g.dx[3];
+      }
+      double hv = 0.0, hx = 0.0;
+      if ((vv <= cc + rc) && (vv >= cc - rc)) hv = cos(PERIOD * ((vv - cc) / rc));
+      else if ((vv <= -cc + rc) && (vv >= -cc - rc)) hv = -cos(PERIOD * ((vv + cc) / rc));
+      if ((xx <= cc + rc) && (xx >= cc - rc)) hx = cos(PERIOD * ((xx - cc) / rc));
+      else if ((xx <= -cc + rc) && (xx >= -cc - rc)) hx = -cos(PERIOD * ((xx + cc) / rc));
+      val = AMPLI * hx * hv;
+      break;
+    }
+    case 10: {  // SLD10, VMO/Init.cpp:226-245
+      const double vy = g.minPhy[3] + ivy * g.dx[3];
+      const double vx = g.minPhy[2] + ivx * g.dx[2];
+      const double y = g.minPhy[1] + iy * g.dx[1];
+      const double x = g.minPhy[0] + ix * g.dx[0];
+      const double sum = (vx * vx + vy * vy);
+      val = (1. / (2 * M_PI)) * exp(-0.5 * (sum)) * (1 + 0.05 * (cos(0.5 * x) * cos(0.5 * y)));
+      break;
+    }
+    case 20: {  // TSI20, VMO/Init.cpp:247-277
+      const double vd = 2.4, vthx = 1.0, vthy = 0.5, alphax = 0.05, alphay = 0.25, kx = 0.2, ky = 0.2;
+      const double vy = g.minPhy[3] + ivy * g.dx[3];
+      const double vx = g.minPhy[2] + ivx * g.dx[2];
+      const double yy = g.minPhy[1] + iy * g.dx[1];
+      const double xx = g.minPhy[0] + ix * g.dx[0];
+      const double a1 = (vx - vd) / vthx, a2 = vy / vthy, a3 = (vx + vd) / vthx;
+      val = 1. / (4 * M_PI * vthx * vthy) * (1 - alphax * sin(kx * xx) - alphay * sin(ky * yy)) *
+            (exp(-.5 * (a1 * a1 + a2 * a2)) + exp(-.5 * (a3 * a3 + a2 * a2)));
+      break;
+    }
+    default:
+      return;
+  }
+  f[t] = (T)val;
+}
+
+template <typename T> void launch_init_case(const Geom &g, T *f, hipStream_t s) {
+  const long n = (long)g.n[0] * g.n[1] * g.n[2] * g.n[3];
+  hipLaunchKernelGGL(k_init_case<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, g);
+}
+
+int spline_variant_bits(const Geom &g);
+
+int kernel_variants(const Geom &g, int precision_bits) {
+  (void)precision_bits;
+  return spline_variant_bits(g) | 4;
+}
+
+#define VLP_INST(T)                                                                                       \
+  template void launch_init_case<T>(const Geom &, T *, hipStream_t);                                      \
+  template void launch_advect2d_rho<T>(const Geom &, const T *, double, double *, int *, hipStream_t);    \
+  template void launch_advect4d<T>(const Geom &, const T *, const double *, const double *, double, T *, \
+                                   double *, double *, int *, hipStream_t);                              \
+  template void launch_moments<T>(const Geom &, const T *, double *, double *, hipStream_t);
+VLP_INST(double)
+VLP_INST(float)
+
+}  // namespace vlp

@@ -2,8 +2,8 @@
 // deck parser, diagnostics and timers (host side).
 //
 // The step is the reference's onetimestep (miniapps/vlp4d_mpi/openmp/
-// Timestep.hpp:34-97) re-planned for one GPU as four streaming sweeps over
-// HBM, each a hand-written kernel (kernels.hip):
+// Timestep.hpp:34-97) re-planned for one GPU as streaming sweeps over HBM,
+// each a hand-written kernel (spline.hip, advect.hip, field.hip):
 //   1. spline x, spline y      f     -> CXY   (halo + alpha-sums computed in-kernel)
 //   2. advect 2-D + rho        CXY   -> rho partials (f_half never stored)
 //   3. Poisson + E             rho   -> E, phi
@@ -38,6 +38,7 @@ struct vlp_ctx {
   size_t esz = 8;
   void *f = nullptr, *fnext = nullptr, *cxy = nullptr, *tv = nullptr, *c4 = nullptr;
   size_t n_f = 0, n_cxy = 0, n_tv = 0, n_c4 = 0;
+  size_t n_c4s = 0;  // stored C4: positions -3..N+2 per dim plus a zeroed tail
   double *rho_part = nullptr, *l2_part = nullptr;
   long n_l2 = 0;
   FieldBufs fb{};
@@ -135,26 +136,41 @@ void free_all(vlp_ctx *c) {
 
 template <typename T> int run_spline_xy(vlp_ctx *c) {
   const Geom &g = c->g;
-  const long N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
-  const long E0 = N0 + 4, E1 = N1 + 4;
+  const long N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const long E0 = g.n[0] + 4, E1 = N1 + 4;
   T *tx = (T *)c->tv;
   vlp::launch_spline_x<T>(g, (const T *)c->f, tx, c->stream);
-  // y-pass: tx (E0, N1, N2*N3) -> cxy (E0, E1, N2*N3)
-  vlp::launch_spline_line<T>(tx, (T *)c->cxy, (int)N1, E0, N2 * N3, E0, E0 * N1, E0, E0 * E1, c->stream);
-  (void)N2;
-  (void)N3;
+  // y-pass: tx (E0, N1, N2, N3) -> cxy (E0, E1, N2, N3); lanes along x
+  vlp::LineMap m{};
+  m.N = (int)N1;
+  m.X0 = E0; m.X1 = N2; m.A = N3;
+  m.is0 = 1; m.is1 = E0 * N1; m.ios = E0 * N1 * N2; m.ils = E0;
+  m.os0 = 1; m.os1 = E0 * E1; m.oos = E0 * E1 * N2; m.ols = E0;
+  vlp::launch_spline_line<T>(tx, (T *)c->cxy, m, c->stream);
   return VLP_OK;
 }
 
 template <typename T> int run_spline_vxvy(vlp_ctx *c) {
   const Geom &g = c->g;
   const long N2 = g.n[2], N3 = g.n[3];
-  const long P = (long)(g.n[0] + 4) * (g.n[1] + 4);
-  const long E2 = N2 + 4;
-  // vx-pass: cxy (P, N2, N3) -> tv (P, E2, N3)
-  vlp::launch_spline_line<T>((const T *)c->cxy, (T *)c->tv, (int)N2, P, N3, P, P * N2, P, P * E2, c->stream);
-  // vy-pass: tv (P*E2, N3) -> c4 (P*E2, E3)
-  vlp::launch_spline_line<T>((const T *)c->tv, (T *)c->c4, (int)N3, P * E2, 1, P * E2, 0, P * E2, 0, c->stream);
+  const long E0 = g.n[0] + 4, E1 = g.n[1] + 4, E2 = N2 + 4;
+  const long F0 = g.n[0] + 6, F1 = g.n[1] + 6, F2 = N2 + 6;
+  const long P = E0 * E1;
+  // vx-pass: cxy (E0*E1, N2, N3) -> tv (E0*E1, E2, N3); lanes along (x, y)
+  vlp::LineMap m{};
+  m.N = (int)N2;
+  m.X0 = P; m.X1 = N3; m.A = 1;
+  m.is0 = 1; m.is1 = P * N2; m.ios = 0; m.ils = P;
+  m.os0 = 1; m.os1 = P * E2; m.oos = 0; m.ols = P;
+  vlp::launch_spline_line<T>((const T *)c->cxy, (T *)c->tv, m, c->stream);
+  // vy-pass: tv (E0, E1, E2, N3) -> c4 (F0, F1, F2, F3) at positions -2..N+1
+  vlp::LineMap q{};
+  q.N = (int)N3;
+  q.X0 = E0; q.X1 = E1; q.A = E2;
+  q.is0 = 1; q.is1 = E0; q.ios = E0 * E1; q.ils = E0 * E1 * E2;
+  q.os0 = 1; q.os1 = F0; q.oos = F0 * F1; q.ols = F0 * F1 * F2;
+  T *c4 = (T *)c->c4 + 1 + F0 * (1 + F1 * (1 + F2 * 1));
+  vlp::launch_spline_line<T>((const T *)c->tv, c4, q, c->stream);
   return VLP_OK;
 }
 
@@ -361,6 +377,7 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
   c->n_cxy = (N0 + 4) * (N1 + 4) * N2 * N3;
   c->n_tv = (N0 + 4) * (N1 + 4) * (N2 + 4) * N3;
   c->n_c4 = (N0 + 4) * (N1 + 4) * (N2 + 4) * (N3 + 4);
+  c->n_c4s = (N0 + 6) * (N1 + 6) * (N2 + 6) * (N3 + 6) + 64;
   c->n_l2 = vlp::l2_partials(c->g);
   const size_t nxh = N0 / 2 + 1;
   auto fail = [&](int code) {
@@ -382,7 +399,7 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
   ALLOC(c->fnext, c->n_f * c->esz);
   ALLOC(c->cxy, c->n_cxy * c->esz);
   ALLOC(c->tv, c->n_tv * c->esz);
-  ALLOC(c->c4, c->n_c4 * c->esz);
+  ALLOC(c->c4, c->n_c4s * c->esz);
   ALLOC(c->rho_part, N0 * N1 * N3 * sizeof(double));
   ALLOC(c->l2_part, (size_t)c->n_l2 * sizeof(double));
   ALLOC(c->fb.rho, N0 * N1 * sizeof(double));
@@ -404,6 +421,7 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
       hipMemcpy(c->fb.twy, twy.data(), twy.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(c->fb.diag, 0, 5 * (size_t)(cfg.nbiter + 1) * sizeof(double)) != hipSuccess ||
       hipMemset(c->d_err, 0, sizeof(int)) != hipSuccess || hipMemset(c->f, 0, c->n_f * c->esz) != hipSuccess ||
+      hipMemset(c->c4, 0, c->n_c4s * c->esz) != hipSuccess ||
       hipMemset(c->fb.rho, 0, N0 * N1 * sizeof(double)) != hipSuccess ||
       hipMemset(c->fb.ex, 0, N0 * N1 * sizeof(double)) != hipSuccess ||
       hipMemset(c->fb.ey, 0, N0 * N1 * sizeof(double)) != hipSuccess)
@@ -529,7 +547,20 @@ int vlp_get_buffer(vlp_ctx *c, int which, double *host, size_t n) {
   switch (which) {
     case VLP_BUF_F: return download(c, c->f, n, host);
     case VLP_BUF_CXY: return download(c, c->cxy, n, host);
-    case VLP_BUF_C4: return download(c, c->c4, n, host);
+    case VLP_BUF_C4: {
+      std::vector<double> full(c->n_c4s);
+      rc = download(c, c->c4, c->n_c4s, full.data());
+      if (rc) return rc;
+      const size_t F0 = c->cfg.nxmax[0] + 6, F1 = c->cfg.nxmax[1] + 6, F2 = c->cfg.nxmax[2] + 6;
+      const size_t E0 = F0 - 2, E1 = F1 - 2, E2 = F2 - 2, E3 = (size_t)c->cfg.nxmax[3] + 4;
+      size_t k = 0;
+      for (size_t i3 = 0; i3 < E3; i3++)
+        for (size_t i2 = 0; i2 < E2; i2++)
+          for (size_t i1 = 0; i1 < E1; i1++)
+            for (size_t i0 = 0; i0 < E0; i0++)
+              host[k++] = full[(i0 + 1) + F0 * ((i1 + 1) + F1 * ((i2 + 1) + F2 * (i3 + 1)))];
+      return VLP_OK;
+    }
     default: break;
   }
   const double *src = which == VLP_BUF_RHO ? c->fb.rho : which == VLP_BUF_EX ? c->fb.ex : c->fb.ey;
@@ -590,5 +621,7 @@ int vlp_timer(vlp_ctx *c, int which, double *seconds, int *calls) {
 const char *vlp_timer_label(int which) { return (which >= 0 && which < VLP_T_COUNT) ? kLabels[which] : ""; }
 
 const char *vlp_last_error(const vlp_ctx *c) { return c ? c->err.c_str() : g_last_error.c_str(); }
+
+int vlp_kernel_variants(const vlp_ctx *c) { return c ? vlp::kernel_variants(c->g, c->precision) : -1; }
 
 }  // extern "C"

@@ -4,13 +4,16 @@
 //   F    : interior f, N0*N1*N2*N3                         (positions 0..N-1 per dim)
 //   CXY  : (N0+4)*(N1+4)*N2*N3, x,y on -2..N+1, v interior  (spline along x and y)
 //   TV   : (N0+4)*(N1+4)*(N2+4)*N3, also holds the x-pass output (N0+4)*N1*N2*N3
-//   C4   : (N0+4)*(N1+4)*(N2+4)*(N3+4), all dims on -2..N+1
-// The reference's (N+6)^4 halo arrays are never materialised: the periodic
-// copies and alpha-sums of its halo exchange are computed inside the spline
-// kernels from the interior line they extend.
+//   C4   : (N0+6)*(N1+6)*(N2+6)*(N3+6), positions -3..N+2 in every dim; the
+//          coefficients fill -2..N+1 and the outer layer stays 0, so every
+//          stencil read of the 4-D advection is in bounds without clamping.
+// The reference's (N+6)^4 halo arrays of f are never materialised: the
+// periodic copies and alpha-sums of its halo exchange are computed inside the
+// spline kernels from the interior line they extend.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
 #include <cstddef>
 #include <cstdint>
 
@@ -26,16 +29,21 @@ struct Geom {
   int idcase;
 };
 
-// Launchers (kernels.hip).  T is the storage type (double or float).
+// Addressing of a strided-line spline pass.  Thread index t -> (i0, i1, o)
+// with i0 fastest; the line of N interior values starts at
+// in + i0*is0 + i1*is1 + o*ios with stride ils; the N+4 coefficients
+// (positions -2..N+1) go to out + i0*os0 + i1*os1 + o*oos with stride ols.
+struct LineMap {
+  int N;
+  long X0, X1, A;
+  long is0, is1, ios, ils;
+  long os0, os1, oos, ols;
+};
+
+// Launchers.  T is the storage type (double or float).
 template <typename T> void launch_init_case(const Geom &g, T *f, hipStream_t s);
 template <typename T> void launch_spline_x(const Geom &g, const T *f, T *tx, hipStream_t s);
-// Strided-line pass: lines of N interior values along a dimension with
-// stride in_ls (input) / out_ls (output); "inner" index i in [0,X) is
-// contiguous (stride 1) in both arrays, "outer" o in [0,A) has strides
-// in_os / out_os.  Output holds positions -2..N+1.
-template <typename T>
-void launch_spline_line(const T *in, T *out, int N, long X, long A, long in_ls, long in_os, long out_ls, long out_os,
-                        hipStream_t s);
+template <typename T> void launch_spline_line(const T *in, T *out, const LineMap &m, hipStream_t s);
 template <typename T>
 void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, double *rho_part, int *err, hipStream_t s);
 template <typename T>
@@ -46,6 +54,12 @@ void launch_moments(const Geom &g, const T *f, double *rho_part, double *l2_part
 
 // Number of l2 partial sums written by launch_advect4d / launch_moments.
 long l2_partials(const Geom &g);
+
+// Kernel variants chosen for a geometry (for tests and reports):
+// bit 0: x-pass uses the multi-lane scan kernel, bit 1: strided passes use
+// the register-line kernel for all three dims, bit 2: 4-D advection uses the
+// vx-blocked kernel.
+int kernel_variants(const Geom &g, int precision_bits);
 
 // Field solve (FP64): rho from the per-vy partial sums, Poisson by DFT with
 // the reference's FFT conventions, field diagnostics; optional L2 norm.

@@ -57,6 +57,7 @@ int main(int argc, char **argv) {
   int rc = vlp_parse_deck(file, &cfg);
   if (rc == VLP_E_FILE) {
     printf("import: error file not found\n");
+    fflush(stdout);
     abort();
   }
   print_config(&cfg);
@@ -64,6 +65,7 @@ int main(int argc, char **argv) {
   rc = vlp_create_config(&cfg, precision, device, &ctx);
   if (rc == VLP_E_CASE) {
     printf("Unknown test case !\n");
+    fflush(stdout);
     abort();
   }
   if (rc) {

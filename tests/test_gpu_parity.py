@@ -17,7 +17,10 @@ from conftest import relmax, sld_box
 pytestmark = pytest.mark.gpu
 
 # (Nx, Ny, Nvx, Nvy) non-uniform; CFL-valid dt (dt*V <= dx).
-SHAPES = [(16, 12, 20, 14), (24, 20, 16, 18)]
+# The first two run every spline pass with one lane per line; the third puts
+# 2-4 lanes on each line (x: 4 lanes of 10, y/vx/vy: 2 lanes) like 64^4 and
+# 128^4 do.
+SHAPES = [(16, 12, 20, 14), (24, 20, 16, 18), (40, 36, 48, 34)]
 
 
 def _configs(vlp, orc, n, idcase=10, dt=0.05, nbiter=8, vmax=6.0, lx=4 * math.pi):
@@ -88,6 +91,7 @@ def test_phases_match_oracle(vlp, orc, n):
 @pytest.mark.parametrize("idcase,n,steps,vmax,lx", [
     (10, (16, 16, 16, 16), 20, 6.0, 4 * math.pi),
     (20, (20, 12, 24, 16), 12, 9.0, 10 * math.pi),
+    (20, (64, 64, 64, 64), 3, 8.0, 10 * math.pi),
 ])
 def test_multistep_matches_oracle(vlp, orc, idcase, n, steps, vmax, lx):
     dt = 0.5 * (lx / n[0]) / vmax
