@@ -9,7 +9,7 @@ and the diagnostics -- the reference's MainLoop timer scope.
 Metric: 4-D cell updates per second = N^4 * steps / wall time of the timed
 steps (inputs resident in HBM).  Prints ONE JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision 64|32]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision 64|32|33]
                   [--n 128] [--no-cpu-baseline]
 """
 import argparse
@@ -110,7 +110,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", type=int, default=64, choices=[64, 32])
+    ap.add_argument("--precision", type=int, default=64, choices=[64, 32, 33],
+                    help="64: FP64; 32: FP32 storage, FP64 interpolation arithmetic; "
+                         "33: FP32 storage and FP32 interpolation arithmetic")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
@@ -122,7 +124,7 @@ def main():
     n = a.n
     nb = a.warmup + a.steps
     cfg = pkg.VlpConfig.make(n, SLD10["minphy"], SLD10["maxphy"], SLD10["idcase"], SLD10["dt"], nb)
-    prec = pkg.FP64 if a.precision == 64 else pkg.FP32
+    prec = {64: pkg.FP64, 32: pkg.FP32, 33: pkg.FP32_ACC}[a.precision]
     b = 8 if a.precision == 64 else 4
     sim = pkg.Vlp4d(cfg, precision=prec)
     if a.warmup:
@@ -157,10 +159,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": (value / A100_REF) if a.precision == 64 and n == 128 else None,
-        "dtype": "f64" if a.precision == 64 else "f32-storage (f64 field solve)",
+        "dtype": {64: "f64", 32: "f32-storage, f64 arithmetic", 33: "f32-storage, f32 interpolation arithmetic"}[
+            a.precision],
         "data": "synthetic: SLD10 Landau-damping initial condition generated on the GPU",
         "config": {"workload": f"vlp4d spline step, SLD10 deck at {n}^4 (box 12.56637112, v in [-9,9], dt 0.01)",
-                   "grid": [n, n, n, n], "idcase": 10, "dt": 0.01, "storage_bits": a.precision},
+                   "grid": [n, n, n, n], "idcase": 10, "dt": 0.01, "storage_bits": 64 if a.precision == 64 else 32},
         "roofline": {"bound": "hbm", "kernel": "k_advect4d", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": adv_bytes, "avg_launch_ms": 1e3 * adv_s},

@@ -38,8 +38,15 @@ enum {
                        VMO/Types.hpp:31), or buffer size mismatch                           */
 };
 
-/* Storage precision of the distribution function and spline coefficients. */
-enum { VLP_FP64 = 64, VLP_FP32 = 32 };
+/* Precision of the distribution function and spline coefficients.
+ * VLP_FP64: stored and interpolated in double (the reference's arithmetic).
+ * VLP_FP32: stored in float; spline recursions, interpolation weights and
+ *   stencil sums in double (storage rounding is then the only FP32 error).
+ * VLP_FP32_ACC: stored in float and interpolated with float weights and float
+ *   accumulation.
+ * Feet, the charge density, the Poisson solve and the diagnostics are double
+ * in every mode. */
+enum { VLP_FP64 = 64, VLP_FP32 = 32, VLP_FP32_ACC = 33 };
 
 /* Domain of VMO/Config.hpp:34-48 for one rank (dx derived as (max-min)/N). */
 typedef struct {
@@ -83,8 +90,8 @@ int vlp_parse_deck(const char *dat_path, vlp_config *cfg);
 
 /* init() of VMO/Init.cpp:324-373 plus the initial field solve and diagnostics
  * of VMO/Vlp4d.cpp:29-34: allocates device state on `device`, runs the init
- * case, solves Poisson, records diagnostics at iter 0.  precision: VLP_FP64 or
- * VLP_FP32.  dat_path variant parses the deck first. */
+ * case, solves Poisson, records diagnostics at iter 0.  precision: VLP_FP64,
+ * VLP_FP32 or VLP_FP32_ACC.  dat_path variant parses the deck first. */
 int vlp_create(const char *dat_path, int precision, int device, vlp_ctx **out);
 int vlp_create_config(const vlp_config *cfg, int precision, int device, vlp_ctx **out);
 
@@ -145,9 +152,11 @@ int vlp_timer(vlp_ctx *ctx, int which, double *seconds, int *calls);
 const char *vlp_timer_label(int which);
 
 /* Which kernel variants this context's geometry selects (bit 0: multi-lane
- * scan for the x spline, bit 1: register-line kernels for the y/vx/vy
- * splines, bit 2: vx-blocked 4-D advection).  No reference counterpart:
- * introspection for tests and reports. */
+ * scan for the x spline, bit 1: multi-lane scan for the y/vx/vy splines,
+ * bit 2: vx-blocked 4-D advection, bit 3: 4-cell blocks (else 2), bit 4: E
+ * rows staged in LDS, bit 5: XCD-aware workgroup order, bit 6: advection
+ * arithmetic in float).  No reference
+ * counterpart: introspection for tests and reports. */
 int vlp_kernel_variants(const vlp_ctx *ctx);
 
 /* Message of the last error on this context (or the last global error when

@@ -22,7 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvlp4d.so")
 
 OK, E_FILE, E_CASE, E_CFL, E_ARG, E_HIP, E_SIZE = range(7)
-FP64, FP32 = 64, 32
+FP64, FP32, FP32_ACC = 64, 32, 33
 BUF_F, BUF_CXY, BUF_C4, BUF_RHO, BUF_EX, BUF_EY = range(6)
 TIMER_LABELS = ["total", "MainLoop", "pack", "comm", "unpack", "advec2D", "advec4D", "field",
                 "all_reduce", "Fourier", "diag", "splinecoeff_xy", "splinecoeff_vxvy"]

@@ -1,7 +1,8 @@
 // advect.hip -- advection, moments and initial-condition kernels (gfx950).
 //
-// Storage type T is double (plain path) or float (FP32 storage); feet and the
-// density/L2 reductions are always computed in double.
+// Storage type T is double (plain path) or float (FP32 storage); A is the type
+// of the interpolation weights and stencil sums (double, or float for float
+// storage on request).  Feet and the density/L2 reductions are always double.
 #include "device_common.hpp"
 
 #include <cstdlib>
@@ -14,10 +15,9 @@ namespace vlp {
 // One lane per (x, y, vy), looping over vx: rho_part[vy][y][x] = sum_vx f_half,
 // the same summation order as field_rho's inner loop.  f_half is not stored.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, typename A>
 __global__ __launch_bounds__(256) void k_advect2d_rho(const T *__restrict__ cxy, Geom g, double dt,
                                                       double *__restrict__ rho_part, int *__restrict__ err) {
-  using A = T;
   const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)N0 * N1 * N3) return;
@@ -63,10 +63,14 @@ __global__ __launch_bounds__(256) void k_advect2d_rho(const T *__restrict__ cxy,
 }
 
 template <typename T>
-void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, double *rho_part, int *err, hipStream_t s) {
+void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, bool arith32, double *rho_part, int *err,
+                         hipStream_t s) {
   const long nthr = (long)g.n[0] * g.n[1] * g.n[3];
-  hipLaunchKernelGGL(k_advect2d_rho<T>, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, cxy, g, dt_half,
-                     rho_part, err);
+  const dim3 grid((unsigned)((nthr + 255) / 256));
+  if (arith32 && sizeof(T) == sizeof(float))
+    hipLaunchKernelGGL((k_advect2d_rho<T, float>), grid, dim3(256), 0, s, cxy, g, dt_half, rho_part, err);
+  else
+    hipLaunchKernelGGL((k_advect2d_rho<T, double>), grid, dim3(256), 0, s, cxy, g, dt_half, rho_part, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -91,32 +95,59 @@ static FeetConsts feet_consts(const Geom &g) {
 }
 #pragma clang fp contract(on)
 
+// E-field sources for compute_foot: (Ex, Ey) at stencil node (jx, k1), jx the
+// wrapped x index, k1 in {0,1,2} the stencil row ip1 + k1 (wrapped).
+struct EGlobal {  // straight from the N1 x N0 arrays
+  const double *__restrict__ ex;
+  const double *__restrict__ ey;
+  int N0, N1, ip1;
+  __device__ __forceinline__ void at(int jx, int k1, double &vx, double &vy) const {
+    const long id = jx + (long)N0 * (((ip1 + k1) % N1 + N1) % N1);
+    vx = ex[id];
+    vy = ey[id];
+  }
+};
+struct ELds {  // the three rows, staged in LDS as [3][N0] per component
+  const double *rx, *ry;
+  int N0;
+  __device__ __forceinline__ void at(int jx, int k1, double &vx, double &vy) const {
+    vx = rx[k1 * N0 + jx];
+    vy = ry[k1 * N0 + jx];
+  }
+};
+
+// Normalised half-step y position (y - dt/2 vy - min) / dy of computeFeet;
+// it depends on (y, vy) only, so a lane computes it once for all its vx.
+__device__ __forceinline__ double half_y(const Geom &g, double y, double vy, double dt) {
+  const double xt1 = y - 0.5 * dt * vy;
+  return g.inv_dx[1] * (xt1 - g.minPhy[1]);
+}
+
 // Characteristic foot of cell (x, y, vx, vy), computeFeet of
 // VMO/Advection.hpp:166-218: half-step position, 3x3 Lagrange interpolation
 // of E with periodic wrap, full-step foot, velocity clamped to the box.
-// Returns false when the foot leaves [min-dx, min+N dx] in any dimension
-// (NaN counts as outside).
-__device__ __forceinline__ bool compute_foot(const Geom &g, const FeetConsts &fc, const double *__restrict__ ex,
-                                             const double *__restrict__ ey, double x, double y, double vx, double vy,
-                                             double dt, double xs[4]) {
-  const int N0 = g.n[0], N1 = g.n[1];
+// xt1 = half_y(...) and ip1 = round(xt1) - 1 come from the caller.  Returns
+// false when the foot leaves [min-dx, min+N dx] in any dimension (NaN counts
+// as outside).
+template <typename ES>
+__device__ __forceinline__ bool compute_foot(const Geom &g, const FeetConsts &fc, const ES &E, double xt1, int ip1,
+                                             double x, double y, double vx, double vy, double dt, double xs[4]) {
+  const int N0 = g.n[0];
   double xt0 = x - 0.5 * dt * vx;
-  double xt1 = y - 0.5 * dt * vy;
   xt0 = g.inv_dx[0] * (xt0 - g.minPhy[0]);
-  xt1 = g.inv_dx[1] * (xt1 - g.minPhy[1]);
   const int ip0 = (int)(round(xt0) - 1);
-  const int ip1 = (int)(round(xt1) - 1);
   const double px = xt0 - ip0, py = xt1 - ip1;
   const double cx[3] = {.5 * (px - 1.) * (px - 2.), -1. * (px) * (px - 2.), .5 * (px) * (px - 1.)};
   const double cy[3] = {.5 * (py - 1.) * (py - 2.), -1. * (py) * (py - 2.), .5 * (py) * (py - 1.)};
   double ftmp1 = 0., ftmp2 = 0.;
   for (int k1 = 0; k1 <= 2; k1++) {
-    const int jy = ((ip1 + k1) % N1 + N1) % N1;  // (N + i) % N, non-negative for any i
     for (int k0 = 0; k0 <= 2; k0++) {
-      const int jx = ((ip0 + k0) % N0 + N0) % N0;
+      const int jx = ((ip0 + k0) % N0 + N0) % N0;  // (N + i) % N, non-negative for any i
       const double cc = cx[k0] * cy[k1];
-      ftmp1 += cc * ex[jx + (long)N0 * jy];
-      ftmp2 += cc * ey[jx + (long)N0 * jy];
+      double e1, e2;
+      E.at(jx, k1, e1, e2);
+      ftmp1 += cc * e1;
+      ftmp2 += cc * e2;
     }
   }
   const double xt2 = vx - 0.5 * dt * ftmp1;
@@ -138,8 +169,32 @@ __device__ __forceinline__ bool compute_foot(const Geom &g, const FeetConsts &fc
   return ok;
 }
 
+// Unaligned vector loads of stencil rows (multi-dword global loads need only
+// dword alignment).
+typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ void load_row4(const double *p, double r[4]) {
+  const d2u a = *(const d2u *)p, b = *(const d2u *)(p + 2);
+  r[0] = a.x;
+  r[1] = a.y;
+  r[2] = b.x;
+  r[3] = b.y;
+}
+__device__ __forceinline__ void load_row4(const float *p, float r[4]) {
+  const f4u a = *(const f4u *)p;
+  r[0] = a.x;
+  r[1] = a.y;
+  r[2] = a.z;
+  r[3] = a.w;
+}
+template <typename T> __device__ __forceinline__ void load_row5(const T *p, T r[5]) {
+  load_row4(p, r);
+  r[4] = p[4];
+}
+
 // Direct 4^4 gather of interp_4D on the C4 layout (positions -3..N+2, stride
-// F_d = N_d + 6).  A valid foot has ip in [-2, N], so positions ip-1..ip+2
+// F_d = N_d + 6).  A valid foot has ip in [-1, N], so positions ip-1..ip+2
 // stay inside -3..N+2; any reading of -3 or N+2 carries weight 0 (or ~1e-16
 // at the lower edge), where the stored value is 0.
 template <typename T, typename A>
@@ -152,14 +207,24 @@ __device__ __forceinline__ A gather4d(const T *__restrict__ c4, long S1, long S2
       A ftmp2 = 0;
       const T *p23 = base + k3 * S3 + k2 * S2;
       for (int k1 = 0; k1 < 4; k1++) {
-        const T *p = p23 + k1 * S1;
+        T r[4];
+        load_row4(p23 + k1 * S1, r);
         A ftmp3 = 0;
-        for (int k0 = 0; k0 < 4; k0++) ftmp3 += eta[0][k0] * (A)p[k0];
+        for (int k0 = 0; k0 < 4; k0++) ftmp3 += eta[0][k0] * (A)r[k0];
         ftmp2 += ftmp3 * eta[1][k1];
       }
       ftmp1 += ftmp2 * eta[2][k2] * eta[3][k3];
     }
   return ftmp1;
+}
+
+// Workgroup order for the 4-D advection.  Workgroups are dealt round-robin to
+// the 8 XCDs (each with its own L2); remapping gives XCD k a contiguous range
+// of logical blocks, so the workgroups resident on one XCD hold neighbouring
+// y rows and share most of their C4 stencil rows in that L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb, int remap) {
+  if (!remap || (nb & 7u)) return b;
+  return (b & 7u) * (nb >> 3) + (b >> 3);
 }
 
 // Lane per (x, y, vy); the lane walks vx in blocks of L cells.  In a block the
@@ -169,27 +234,55 @@ __device__ __forceinline__ A gather4d(const T *__restrict__ c4, long S1, long S2
 // vx-slices -- and every row feeds up to 4 cells (x offset folded into a
 // zero-padded 5-weight vector).  Blocks that do not have this structure, and
 // the tail block, fall back to the direct gather.
-template <typename T, int L>
+//
+// ELDS (requires N0 % 64 == 0, so that every wave has one (y, vy)): the wave
+// stages the three E rows its feet interpolate from in LDS once, instead of
+// 18 gathered global loads per cell.
+template <typename T, typename A, int L, bool ELDS>
 __global__ __launch_bounds__(256) void k_advect4d_blk(const T *__restrict__ c4, const double *__restrict__ ex,
                                                       const double *__restrict__ ey, Geom g, FeetConsts fc, double dt,
-                                                      T *__restrict__ fout, double *__restrict__ rho_part,
+                                                      int remap, T *__restrict__ fout, double *__restrict__ rho_part,
                                                       double *__restrict__ l2_part, int *__restrict__ err) {
-  using A = T;
   __shared__ double sh[8];
+  extern __shared__ double esh[];  // ELDS: [wave][component][3][N0]
   const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x, remap);
+  const long t = (long)lb * blockDim.x + threadIdx.x;
   const bool active = t < (long)N0 * N1 * N3;
+  const int ix = (int)(t % N0);
+  const long r = t / N0;
+  const int iy = (int)(r % N1);
+  const int ivy = (int)(r / N1);
+  const double y = g.minPhy[1] + iy * g.dx[1];
+  const double vy = g.minPhy[3] + ivy * g.dx[3];
+  const double xt1 = half_y(g, y, vy, dt);
+  int ip1 = (int)(round(xt1) - 1);
+  ELds el;
+  if (ELDS) {
+    // (y, vy) and hence ip1 are uniform in the wave; a wave past the end
+    // stages nothing and only joins the barrier.
+    ip1 = __builtin_amdgcn_readfirstlane(ip1);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *rx = esh + (long)w * 6 * N0;
+    el.rx = rx;
+    el.ry = rx + 3 * N0;
+    el.N0 = N0;
+    if (active) {
+      for (int q = lane; q < 3 * N0; q += 64) {
+        const int k1 = q / N0, jx = q - k1 * N0;
+        const long id = jx + (long)N0 * (((ip1 + k1) % N1 + N1) % N1);
+        rx[q] = ex[id];
+        rx[3 * N0 + q] = ey[id];
+      }
+    }
+    __syncthreads();
+  }
+  const EGlobal eg{ex, ey, N0, N1, ip1};
   double l2 = 0.;
   int bad = 0;
   if (active) {
-    const int ix = (int)(t % N0);
-    const long r = t / N0;
-    const int iy = (int)(r % N1);
-    const int ivy = (int)(r / N1);
     const long S1 = N0 + 6, S2 = S1 * (N1 + 6), S3 = S2 * (N2 + 6);
     const double x = g.minPhy[0] + ix * g.dx[0];
-    const double y = g.minPhy[1] + iy * g.dx[1];
-    const double vy = g.minPhy[3] + ivy * g.dx[3];
     T *orow = fout + ix + (long)N0 * (iy + (long)N1 * (long)N2 * ivy);
     const long ostride = (long)N0 * N1;
     double rho = 0.;
@@ -206,7 +299,8 @@ __global__ __launch_bounds__(256) void k_advect4d_blk(const T *__restrict__ c4, 
         if (ivx < N2) {
           const double vx = g.minPhy[2] + ivx * g.dx[2];
           double xs[4];
-          ok[c] = compute_foot(g, fc, ex, ey, x, y, vx, vy, dt, xs);
+          if (ELDS) ok[c] = compute_foot(g, fc, el, xt1, ip1, x, y, vx, vy, dt, xs);
+          else ok[c] = compute_foot(g, fc, eg, xt1, ip1, x, y, vx, vy, dt, xs);
           if (ok[c]) {
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -246,6 +340,9 @@ __global__ __launch_bounds__(256) void k_advect4d_blk(const T *__restrict__ c4, 
           }
           val[c] = 0;
         }
+        // first stencil position bx0-1 sits at storage index bx0+2; the 5th
+        // value (position bx0+3, at most N0+3) may be the next row's first
+        // entry -- finite, and weighted 0 in that case.
         const T *base = c4 + (bx0 + 2) + S1 * (ip[0][1] + 2) + S2 * (ip[0][2] + 2) + S3 * (long)(ip[0][3] + 2);
 #pragma unroll
         for (int j = 0; j < L + 3; j++) {
@@ -256,17 +353,15 @@ __global__ __launch_bounds__(256) void k_advect4d_blk(const T *__restrict__ c4, 
             for (int c = 0; c < L; c++) tk[c] = 0;
 #pragma unroll
             for (int k1 = 0; k1 < 4; k1++) {
-              const T *rowp = base + j * S2 + k3 * S3 + k1 * S1;
-              A rv[5];
-#pragma unroll
-              for (int i = 0; i < 5; i++) rv[i] = (A)rowp[i];
+              T rv[5];
+              load_row5(base + j * S2 + k3 * S3 + k1 * S1, rv);
 #pragma unroll
               for (int c = 0; c < L; c++) {
                 const int k2 = j - c;
                 if (k2 >= 0 && k2 < 4) {
                   A s = 0;
 #pragma unroll
-                  for (int i = 0; i < 5; i++) s += e0[c][i] * rv[i];
+                  for (int i = 0; i < 5; i++) s += e0[c][i] * (A)rv[i];
                   tk[c] += s * eta[c][0][k1];
                 }
               }
@@ -307,34 +402,75 @@ __global__ __launch_bounds__(256) void k_advect4d_blk(const T *__restrict__ c4, 
     rho_part[ix + (long)N0 * (iy + (long)N1 * ivy)] = rho;
   }
   const double tot = block_sum(l2, sh);
-  if (threadIdx.x == 0) l2_part[blockIdx.x] = tot;
+  if (threadIdx.x == 0) l2_part[lb] = tot;
   if (bad) atomicOr(err, 2);
 }
 
 long l2_partials(const Geom &g) { return ((long)g.n[0] * g.n[1] * g.n[3] + 255) / 256; }
 
-// Cells per lane block in the 4-D advection (register blocking along vx).
-// Tuning knob: VLP4D_ADVECT_L=2|4 overrides the default.
-static int advect_block(int bits) {
-  static int env = -1;
-  if (env < 0) {
-    const char *e = getenv("VLP4D_ADVECT_L");
-    env = e ? atoi(e) : 0;
+// Tuning knobs of the 4-D advection, read from the environment at each launch
+// (a getenv scan, negligible next to the kernel):
+// VLP4D_ADVECT_L=2|4 cells per lane block along vx (default: 4, or 2 for float
+//   storage with double arithmetic, whose 4-cell block needs more than 256
+//   VGPRs and would run one wave per SIMD),
+// VLP4D_XCD_REMAP=0 keeps the hardware workgroup order,
+// VLP4D_ELDS=0 reads E from global memory even when N0 % 64 == 0.
+struct AdvectTuning {
+  int L = 0, remap = 1, elds = 1;  // L = 0: per-type default
+};
+static AdvectTuning advect_tuning() {
+  AdvectTuning a;
+  if (const char *e = getenv("VLP4D_ADVECT_L")) {
+    const int v = atoi(e);
+    if (v == 2 || v == 4) a.L = v;
   }
-  if (env == 2 || env == 4) return env;
-  return bits == 64 ? 4 : 4;
+  if (const char *e = getenv("VLP4D_XCD_REMAP")) a.remap = atoi(e) != 0;
+  if (const char *e = getenv("VLP4D_ELDS")) a.elds = atoi(e) != 0;
+  return a;
+}
+static int advect_L(bool mixed) {
+  const int L = advect_tuning().L;
+  return L ? L : (mixed ? 2 : 4);
 }
 
-template <typename T>
-void launch_advect4d(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, T *fout,
-                     double *rho_part, double *l2_part, int *err, hipStream_t s) {
-  const dim3 grid((unsigned)l2_partials(g));
-  if (advect_block(8 * (int)sizeof(T)) == 2)
-    hipLaunchKernelGGL((k_advect4d_blk<T, 2>), grid, dim3(256), 0, s, c4, ex, ey, g, feet_consts(g), dt, fout,
-                       rho_part, l2_part, err);
-  else
-    hipLaunchKernelGGL((k_advect4d_blk<T, 4>), grid, dim3(256), 0, s, c4, ex, ey, g, feet_consts(g), dt, fout,
-                       rho_part, l2_part, err);
+// LDS for the staged E rows: 4 waves x 2 components x 3 rows x N0 doubles.
+static bool use_elds(const Geom &g) { return advect_tuning().elds && g.n[0] % 64 == 0 && g.n[0] <= 512; }
+
+template <typename T, typename A, int L, bool ELDS>
+static void launch_blk(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, T *fout,
+                       double *rho_part, double *l2_part, int *err, hipStream_t s) {
+  const size_t lds = ELDS ? (size_t)4 * 6 * g.n[0] * sizeof(double) : 0;
+  if (ELDS && lds > 48 * 1024)
+    (void)hipFuncSetAttribute((const void *)k_advect4d_blk<T, A, L, ELDS>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_advect4d_blk<T, A, L, ELDS>), dim3((unsigned)l2_partials(g)), dim3(256), lds, s, c4, ex, ey,
+                     g, feet_consts(g), dt, advect_tuning().remap, fout, rho_part, l2_part, err);
+}
+
+template <typename T, typename A>
+static void launch_advect4d_a(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, T *fout,
+                              double *rho_part, double *l2_part, int *err, hipStream_t s) {
+  const bool el = use_elds(g);
+  if (advect_L(sizeof(T) != sizeof(A)) == 2) {
+    if (el) launch_blk<T, A, 2, true>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
+    else launch_blk<T, A, 2, false>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
+  } else {
+    if (el) launch_blk<T, A, 4, true>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
+    else launch_blk<T, A, 4, false>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
+  }
+}
+
+template <>
+void launch_advect4d<double>(const Geom &g, const double *c4, const double *ex, const double *ey, double dt, bool,
+                             double *fout, double *rho_part, double *l2_part, int *err, hipStream_t s) {
+  launch_advect4d_a<double, double>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
+}
+
+template <>
+void launch_advect4d<float>(const Geom &g, const float *c4, const double *ex, const double *ey, double dt,
+                            bool arith32, float *fout, double *rho_part, double *l2_part, int *err, hipStream_t s) {
+  if (arith32) launch_advect4d_a<float, float>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
+  else launch_advect4d_a<float, double>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
 }
 
 // moments of a given f: rho partials (sum over vx) and L2 partials
@@ -436,16 +572,17 @@ template <typename T> void launch_init_case(const Geom &g, T *f, hipStream_t s) 
 
 int spline_variant_bits(const Geom &g);
 
-int kernel_variants(const Geom &g, int precision_bits) {
-  (void)precision_bits;
-  return spline_variant_bits(g) | 4;
+int kernel_variants(const Geom &g, bool fp32_storage, bool arith32) {
+  const AdvectTuning t = advect_tuning();
+  const bool remap = t.remap && (l2_partials(g) % 8 == 0);
+  const int L = advect_L(fp32_storage && !arith32);
+  return spline_variant_bits(g) | 4 | (L == 4 ? 8 : 0) | (use_elds(g) ? 16 : 0) | (remap ? 32 : 0) |
+         (arith32 ? 64 : 0);
 }
 
-#define VLP_INST(T)                                                                                       \
-  template void launch_init_case<T>(const Geom &, T *, hipStream_t);                                      \
-  template void launch_advect2d_rho<T>(const Geom &, const T *, double, double *, int *, hipStream_t);    \
-  template void launch_advect4d<T>(const Geom &, const T *, const double *, const double *, double, T *, \
-                                   double *, double *, int *, hipStream_t);                              \
+#define VLP_INST(T)                                                                                        \
+  template void launch_init_case<T>(const Geom &, T *, hipStream_t);                                       \
+  template void launch_advect2d_rho<T>(const Geom &, const T *, double, bool, double *, int *, hipStream_t); \
   template void launch_moments<T>(const Geom &, const T *, double *, double *, hipStream_t);
 VLP_INST(double)
 VLP_INST(float)

@@ -175,12 +175,13 @@ template <typename T> int run_spline_vxvy(vlp_ctx *c) {
 }
 
 template <typename T> void run_advect2d(vlp_ctx *c) {
-  vlp::launch_advect2d_rho<T>(c->g, (const T *)c->cxy, 0.5 * c->cfg.dt, c->rho_part, c->d_err, c->stream);
+  vlp::launch_advect2d_rho<T>(c->g, (const T *)c->cxy, 0.5 * c->cfg.dt, c->precision == VLP_FP32_ACC, c->rho_part,
+                              c->d_err, c->stream);
 }
 
 template <typename T> void run_advect4d(vlp_ctx *c) {
-  vlp::launch_advect4d<T>(c->g, (const T *)c->c4, c->fb.ex, c->fb.ey, c->cfg.dt, (T *)c->fnext, c->rho_part,
-                          c->l2_part, c->d_err, c->stream);
+  vlp::launch_advect4d<T>(c->g, (const T *)c->c4, c->fb.ex, c->fb.ey, c->cfg.dt, c->precision == VLP_FP32_ACC,
+                          (T *)c->fnext, c->rho_part, c->l2_part, c->d_err, c->stream);
 }
 
 template <typename T> void run_moments(vlp_ctx *c) {
@@ -350,7 +351,8 @@ int vlp_create(const char *dat_path, int precision, int device, vlp_ctx **out) {
 int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_ctx **out) {
   if (!cfg_in || !out) return set_err(nullptr, VLP_E_ARG, "null argument");
   *out = nullptr;
-  if (precision != VLP_FP64 && precision != VLP_FP32) return set_err(nullptr, VLP_E_ARG, "precision must be 64 or 32");
+  if (precision != VLP_FP64 && precision != VLP_FP32 && precision != VLP_FP32_ACC)
+    return set_err(nullptr, VLP_E_ARG, "precision must be VLP_FP64 (64), VLP_FP32 (32) or VLP_FP32_ACC (33)");
   vlp_config cfg = *cfg_in;
   derive_dx(&cfg);
   int rc = check_config(&cfg);
@@ -622,6 +624,8 @@ const char *vlp_timer_label(int which) { return (which >= 0 && which < VLP_T_COU
 
 const char *vlp_last_error(const vlp_ctx *c) { return c ? c->err.c_str() : g_last_error.c_str(); }
 
-int vlp_kernel_variants(const vlp_ctx *c) { return c ? vlp::kernel_variants(c->g, c->precision) : -1; }
+int vlp_kernel_variants(const vlp_ctx *c) {
+  return c ? vlp::kernel_variants(c->g, c->precision != VLP_FP64, c->precision == VLP_FP32_ACC) : -1;
+}
 
 }  // extern "C"

@@ -40,26 +40,30 @@ struct LineMap {
   long os0, os1, oos, ols;
 };
 
-// Launchers.  T is the storage type (double or float).
+// Launchers.  T is the storage type (double or float); arith32 selects float
+// interpolation weights and accumulation for float storage (else double).
 template <typename T> void launch_init_case(const Geom &g, T *f, hipStream_t s);
 template <typename T> void launch_spline_x(const Geom &g, const T *f, T *tx, hipStream_t s);
 template <typename T> void launch_spline_line(const T *in, T *out, const LineMap &m, hipStream_t s);
 template <typename T>
-void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, double *rho_part, int *err, hipStream_t s);
+void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, bool arith32, double *rho_part, int *err,
+                        hipStream_t s);
 template <typename T>
-void launch_advect4d(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, T *fout,
-                     double *rho_part, double *l2_part, int *err, hipStream_t s);
+void launch_advect4d(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, bool arith32,
+                     T *fout, double *rho_part, double *l2_part, int *err, hipStream_t s);
 template <typename T>
 void launch_moments(const Geom &g, const T *f, double *rho_part, double *l2_part, hipStream_t s);
 
 // Number of l2 partial sums written by launch_advect4d / launch_moments.
 long l2_partials(const Geom &g);
 
-// Kernel variants chosen for a geometry (for tests and reports):
-// bit 0: x-pass uses the multi-lane scan kernel, bit 1: strided passes use
-// the register-line kernel for all three dims, bit 2: 4-D advection uses the
-// vx-blocked kernel.
-int kernel_variants(const Geom &g, int precision_bits);
+// Kernel variants chosen for a geometry and precision mode (for tests and
+// reports): bit 0: x-pass uses the multi-lane scan kernel, bit 1: strided
+// passes use the multi-lane scan kernel for all three dims, bit 2: 4-D
+// advection uses the vx-blocked kernel, bit 3: 4-cell blocks (else 2), bit 4:
+// E rows staged in LDS, bit 5: XCD-aware workgroup order, bit 6: float
+// advection arithmetic.
+int kernel_variants(const Geom &g, bool fp32_storage, bool arith32);
 
 // Field solve (FP64): rho from the per-vy partial sums, Poisson by DFT with
 // the reference's FFT conventions, field diagnostics; optional L2 norm.

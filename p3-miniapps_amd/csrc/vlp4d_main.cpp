@@ -9,7 +9,9 @@
 // too-large time step prints its message and exits with status 0.
 //
 // Extra flags: "--fp32" selects FP32 storage of f and the spline
-// coefficients, "-d N" the device (as the reference's --device).
+// coefficients (double interpolation arithmetic), "--fp32-acc" FP32 storage
+// with float interpolation arithmetic, "-d N" the device (as the reference's
+// --device).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -51,6 +53,7 @@ int main(int argc, char **argv) {
     if ((!strcmp(argv[i], "-f") || !strcmp(argv[i], "--file")) && i + 1 < argc) file = argv[++i];
     else if ((!strcmp(argv[i], "-d") || !strcmp(argv[i], "--device")) && i + 1 < argc) device = atoi(argv[++i]);
     else if (!strcmp(argv[i], "--fp32")) precision = VLP_FP32;
+    else if (!strcmp(argv[i], "--fp32-acc")) precision = VLP_FP32_ACC;
   }
   printf("reading input file %s\n", file);
   vlp_config cfg;

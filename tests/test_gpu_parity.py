@@ -122,3 +122,30 @@ def test_cfl_violation_reports_error(vlp):
         with pytest.raises(vlp.Vlp4dError) as ei:
             sim.step(1)
         assert ei.value.code == vlp.E_CFL
+
+
+# The 4-D advection has tuned variants picked at launch from the environment
+# (cells per lane block, E rows staged in LDS when Nx % 64 == 0, XCD-aware
+# workgroup order).  Every combination must give the oracle's result; the
+# first shape takes the LDS path, the second cannot.
+@pytest.mark.parametrize("n", [(64, 20, 24, 16), (40, 36, 48, 34)])
+@pytest.mark.parametrize("L,elds,remap", [(4, 1, 1), (4, 0, 0), (2, 1, 0), (2, 0, 1)])
+def test_advect4d_variants_match_oracle(vlp, orc, monkeypatch, n, L, elds, remap):
+    monkeypatch.setenv("VLP4D_ADVECT_L", str(L))
+    monkeypatch.setenv("VLP4D_ELDS", str(elds))
+    monkeypatch.setenv("VLP4D_XCD_REMAP", str(remap))
+    steps = 2
+    dt = 0.5 * (4 * math.pi / n[0]) / 6.0
+    gc, oc = _configs(vlp, orc, n, dt=dt, nbiter=steps)
+    st = orc.State(oc)
+    st.step(steps)
+    with vlp.Vlp4d(gc) as sim:
+        bits = sim.kernel_variants()
+        sim.step(steps)
+        f = sim.get_f()
+        d = sim.diags()
+    assert bool(bits & 8) == (L == 4)
+    assert bool(bits & 32) == (remap == 1)
+    assert bool(bits & 16) == (elds == 1 and n[0] % 64 == 0)
+    assert relmax(f, st.f_interior()) < 1e-12
+    assert relmax(d["nrj"], st.diags()["nrj"]) < 1e-12
