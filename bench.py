@@ -11,6 +11,8 @@ steps (inputs resident in HBM).  Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision 64|32|33]
                   [--n 128] [--no-cpu-baseline]
+  python bench.py --calibrate-copy     # plain copy kernel of known size, for
+                                       # scaling rocprofv3 FETCH_SIZE/WRITE_SIZE
 """
 import argparse
 import json
@@ -26,6 +28,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 
 METRIC = "4D cell updates/s at 128^4 (FP32-store vs FP64 GPU) + % of HBM roofline"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TFS = 78.6   # FP64 vector (= non-packed FP32) peak of MI355X
+# Algorithmic flops of one interp_4D + computeFeet per cell, counted from the
+# reference loops (VMO/Advection.hpp:166-218, 221-296): 336 FMA + 16 MUL in
+# the contraction, 56 for the 4x4 weights, ~100 for the feet.
+ADVECT4D_FLOP_PER_CELL = 860
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "advect4d_traffic_sld10_128.txt")
 A100_REF = 2.38e9  # 40*128^4 / 4.51995 s, A100 stdpar MainLoop (docs/vlp4d.md:148,168-171)
 SLD10 = dict(minphy=[0.0, 0.0, -9.0, -9.0], maxphy=[12.56637112, 12.56637112, 9.0, 9.0], idcase=10, dt=0.01)
 
@@ -82,6 +90,35 @@ def advect4d_bytes(n, b):
     return b * ((n + 4) ** 4 + n ** 4)
 
 
+def measured_traffic(precision, n):
+    """HBM bytes per k_advect4d_blk launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when no summary for this precision and size exists."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        m = re.match(r"^TRAFFIC precision=(\d+) n=(\d+) bytes_per_launch=([0-9.eE+]+)", line)
+        if m and int(m.group(1)) == precision and int(m.group(2)) == n:
+            return float(m.group(3))
+    return None
+
+
+def calibrate_copy():
+    """A plain copy kernel of known size (torch's elementwise copy of 2^27
+    doubles: 1 GiB read + 1 GiB written per launch), run 3 times."""
+    import torch
+    x = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 3
+    print(json.dumps({"copy_kernel": "torch.mul(x, 1.0, out=y)", "read_bytes": 2 ** 30, "write_bytes": 2 ** 30,
+                      "launches": 3, "GBps": 2 * 2 ** 30 / dt / 1e9}))
+
+
 def cpu_baseline(n, steps=1):
     """Oracle (C/OpenMP port of the reference OpenMP step) on the host cores,
     one MainLoop step of the same deck.  Bounded sample: `steps` step(s)."""
@@ -115,11 +152,15 @@ def main():
                          "33: FP32 storage and FP32 interpolation arithmetic")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
               "the multi-GPU decomposition is a design note, not an implemented path.", file=sys.stderr)
         sys.exit(2)
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
     pkg = load_pkg()
     n = a.n
     nb = a.warmup + a.steps
@@ -145,6 +186,7 @@ def main():
     adv_s = (t_after["advec4D"][0] - t_before["advec4D"][0]) / a.steps
     adv_bytes = advect4d_bytes(n, b)
     achieved = adv_bytes / adv_s / 1e9
+    valu = ADVECT4D_FLOP_PER_CELL * n ** 4 / adv_s / 1e12
     d = sim.diags()
     sim.close()
     step_bytes = algorithmic_bytes(n, b)
@@ -164,9 +206,11 @@ def main():
         "data": "synthetic: SLD10 Landau-damping initial condition generated on the GPU",
         "config": {"workload": f"vlp4d spline step, SLD10 deck at {n}^4 (box 12.56637112, v in [-9,9], dt 0.01)",
                    "grid": [n, n, n, n], "idcase": 10, "dt": 0.01, "storage_bits": 64 if a.precision == 64 else 32},
-        "roofline": {"bound": "hbm", "kernel": "k_advect4d", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_launch": adv_bytes, "avg_launch_ms": 1e3 * adv_s},
+        "roofline": {"bound": "hbm", "kernel": "k_advect4d_blk", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(a.precision, n),
+                     "algorithmic_bytes_per_launch": adv_bytes, "avg_launch_ms": 1e3 * adv_s,
+                     "valu": {"achieved": valu, "peak": VALU_PEAK_TFS, "unit": "TFLOP/s",
+                              "frac": valu / VALU_PEAK_TFS, "flop_per_cell": ADVECT4D_FLOP_PER_CELL}},
         "step_hbm_fraction": step_bytes / (dt / a.steps) / 1e9 / HBM_PEAK_GBS,
         "step_algorithmic_bytes": step_bytes,
         "phases_ms": phases,

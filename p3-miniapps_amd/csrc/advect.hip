@@ -221,7 +221,9 @@ __device__ __forceinline__ A gather4d(const T *__restrict__ c4, long S1, long S2
 // Workgroup order for the 4-D advection.  Workgroups are dealt round-robin to
 // the 8 XCDs (each with its own L2); remapping gives XCD k a contiguous range
 // of logical blocks, so the workgroups resident on one XCD hold neighbouring
-// y rows and share most of their C4 stencil rows in that L2.
+// y rows and share most of their C4 stencil rows in that L2.  Without it the
+// workgroups in flight on the whole chip hold one compact range of (y, vy)
+// rows, which the shared last-level cache holds instead.
 __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb, int remap) {
   if (!remap || (nb & 7u)) return b;
   return (b & 7u) * (nb >> 3) + (b >> 3);
@@ -413,10 +415,12 @@ long l2_partials(const Geom &g) { return ((long)g.n[0] * g.n[1] * g.n[3] + 255) 
 // VLP4D_ADVECT_L=2|4 cells per lane block along vx (default: 4, or 2 for float
 //   storage with double arithmetic, whose 4-cell block needs more than 256
 //   VGPRs and would run one wave per SIMD),
-// VLP4D_XCD_REMAP=0 keeps the hardware workgroup order,
+// VLP4D_XCD_REMAP=1 gives each XCD a contiguous range of (y, vy) rows (off by
+//   default: at 128^4 it spreads the concurrently read C4 planes over more
+//   than the 256 MB last-level cache and measured ~12% slower),
 // VLP4D_ELDS=0 reads E from global memory even when N0 % 64 == 0.
 struct AdvectTuning {
-  int L = 0, remap = 1, elds = 1;  // L = 0: per-type default
+  int L = 0, remap = 0, elds = 1;  // L = 0: per-type default
 };
 static AdvectTuning advect_tuning() {
   AdvectTuning a;
