@@ -412,15 +412,16 @@ long l2_partials(const Geom &g) { return ((long)g.n[0] * g.n[1] * g.n[3] + 255) 
 
 // Tuning knobs of the 4-D advection, read from the environment at each launch
 // (a getenv scan, negligible next to the kernel):
-// VLP4D_ADVECT_L=2|4 cells per lane block along vx (default: 4, or 2 for float
-//   storage with double arithmetic, whose 4-cell block needs more than 256
-//   VGPRs and would run one wave per SIMD),
+// VLP4D_ADVECT_L=2|4 cells per lane block along vx (default 2: measured at
+//   128^4 it beats 4 in every precision mode -- FP64 21.3 vs 23.0 ms -- as the
+//   smaller block keeps three waves per SIMD; float storage with double
+//   arithmetic needs more than 256 VGPRs at 4 and runs one wave per SIMD),
 // VLP4D_XCD_REMAP=1 gives each XCD a contiguous range of (y, vy) rows (off by
 //   default: at 128^4 it spreads the concurrently read C4 planes over more
 //   than the 256 MB last-level cache and measured ~12% slower),
 // VLP4D_ELDS=0 reads E from global memory even when N0 % 64 == 0.
 struct AdvectTuning {
-  int L = 0, remap = 0, elds = 1;  // L = 0: per-type default
+  int L = 0, remap = 0, elds = 1;  // L = 0: default block
 };
 static AdvectTuning advect_tuning() {
   AdvectTuning a;
@@ -432,9 +433,9 @@ static AdvectTuning advect_tuning() {
   if (const char *e = getenv("VLP4D_ELDS")) a.elds = atoi(e) != 0;
   return a;
 }
-static int advect_L(bool mixed) {
+static int advect_L() {
   const int L = advect_tuning().L;
-  return L ? L : (mixed ? 2 : 4);
+  return L ? L : 2;
 }
 
 // LDS for the staged E rows: 4 waves x 2 components x 3 rows x N0 doubles.
@@ -455,7 +456,7 @@ template <typename T, typename A>
 static void launch_advect4d_a(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, T *fout,
                               double *rho_part, double *l2_part, int *err, hipStream_t s) {
   const bool el = use_elds(g);
-  if (advect_L(sizeof(T) != sizeof(A)) == 2) {
+  if (advect_L() == 2) {
     if (el) launch_blk<T, A, 2, true>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
     else launch_blk<T, A, 2, false>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
   } else {
@@ -579,7 +580,8 @@ int spline_variant_bits(const Geom &g);
 int kernel_variants(const Geom &g, bool fp32_storage, bool arith32) {
   const AdvectTuning t = advect_tuning();
   const bool remap = t.remap && (l2_partials(g) % 8 == 0);
-  const int L = advect_L(fp32_storage && !arith32);
+  (void)fp32_storage;
+  const int L = advect_L();
   return spline_variant_bits(g) | 4 | (L == 4 ? 8 : 0) | (use_elds(g) ? 16 : 0) | (remap ? 32 : 0) |
          (arith32 ? 64 : 0);
 }
