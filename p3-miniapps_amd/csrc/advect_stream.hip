@@ -1,0 +1,503 @@
+// advect_stream.hip -- y-streaming 4-D advection (gfx950).
+//
+// Same arithmetic as advect_4D of the reference (computeFeet
+// VMO/Advection.hpp:166-218, interp_4D :221-296, the loop of :409-498); the
+// work is organised so that every C4 coefficient a workgroup needs crosses
+// from L2 into the CU once, through LDS.
+//
+// Workgroup = (all x) x (TVX consecutive vx) at one vy.  It walks y: at step q
+// it holds one y-plane of C4 in LDS -- positions x -2..N0+1, vx tile-2 ..
+// tile+TVX+1, vy ivy-2..ivy+2 (a slab) -- with the next two in flight.
+// Lane (x, vx) keeps the four cells (x, y, vx, vy) whose y-stencil contains
+// the current plane "open": their B-spline weights along x, y, vx, vy and an
+// accumulator.  For vy > 0 every regular foot has floor(y*) = y-1 (for vy <= 0,
+// y), so cell y reads planes y+oy-1 .. y+oy+2 with oy fixed per workgroup, is
+// opened when its first plane arrives and written after its fourth.  Each 4-wide
+// x row read from LDS then feeds all four open cells: 16 rows x 4 values per
+// step and per lane, 340 FMAs per cell, no re-reads from L2.
+//
+// A lane reads one set of rows for its four cells when they agree on their x,
+// vx and vy stencil offsets (each in {-1, 0} relative to the cell), else each
+// cell reads its own rows.  A foot outside this pattern (displacement beyond
+// one cell, or y offset different from the workgroup's) is interpolated by the
+// direct 4^4 gather from C4 in global memory at the step it is opened; its
+// streamed weights are zero.
+#include "advect_common.hpp"
+
+#include <cstdlib>
+
+namespace vlp {
+
+namespace {
+
+// One open cell: weights [dim][k] (dims x, y, vx, vy), running sum, and the
+// packed stencil offsets (ox+1) | (ovx+1) << 1 | (ovy+1) << 2, or -1 when the
+// cell takes no streamed contribution.
+template <typename A> struct OpenCell {
+  A w[4][4];
+  A acc;
+  int key;
+};
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// s_waitcnt vmcnt(n) for a wave-uniform n (the count is an immediate of the
+// instruction); n above 63 waits for all.
+#define VLP_VMWAIT(N) \
+  case N: __builtin_amdgcn_s_waitcnt(0x0F70 | ((N) & 15) | (((N) >> 4) << 14)); break;
+__device__ __forceinline__ void wait_vm(int n) {
+  switch (n) {
+    VLP_VMWAIT(1) VLP_VMWAIT(2) VLP_VMWAIT(3) VLP_VMWAIT(4) VLP_VMWAIT(5) VLP_VMWAIT(6) VLP_VMWAIT(7)
+    VLP_VMWAIT(8) VLP_VMWAIT(9) VLP_VMWAIT(10) VLP_VMWAIT(11) VLP_VMWAIT(12) VLP_VMWAIT(13) VLP_VMWAIT(14)
+    VLP_VMWAIT(15) VLP_VMWAIT(16) VLP_VMWAIT(17) VLP_VMWAIT(18) VLP_VMWAIT(19) VLP_VMWAIT(20) VLP_VMWAIT(21)
+    VLP_VMWAIT(22) VLP_VMWAIT(23) VLP_VMWAIT(24) VLP_VMWAIT(25) VLP_VMWAIT(26) VLP_VMWAIT(27) VLP_VMWAIT(28)
+    VLP_VMWAIT(29) VLP_VMWAIT(30) VLP_VMWAIT(31) VLP_VMWAIT(32) VLP_VMWAIT(33) VLP_VMWAIT(34) VLP_VMWAIT(35)
+    VLP_VMWAIT(36) VLP_VMWAIT(37) VLP_VMWAIT(38) VLP_VMWAIT(39) VLP_VMWAIT(40) VLP_VMWAIT(41) VLP_VMWAIT(42)
+    VLP_VMWAIT(43) VLP_VMWAIT(44) VLP_VMWAIT(45) VLP_VMWAIT(46) VLP_VMWAIT(47) VLP_VMWAIT(48) VLP_VMWAIT(49)
+    VLP_VMWAIT(50) VLP_VMWAIT(51) VLP_VMWAIT(52) VLP_VMWAIT(53) VLP_VMWAIT(54) VLP_VMWAIT(55) VLP_VMWAIT(56)
+    VLP_VMWAIT(57) VLP_VMWAIT(58) VLP_VMWAIT(59) VLP_VMWAIT(60) VLP_VMWAIT(61) VLP_VMWAIT(62) VLP_VMWAIT(63)
+    default: __builtin_amdgcn_s_waitcnt(0x0F70); break;
+  }
+}
+#undef VLP_VMWAIT
+
+// Per-workgroup constants and per-lane state of the streaming loop.  The slab
+// is kept in the storage type T (moved by LDS-DMA, 16 bytes per lane) and
+// converted to the arithmetic type A as it is read.  E rows sit in LDS as
+// (Ex, Ey) pairs: [buffer][row k1 = 0..2][x].
+//
+// computeFeet's 3x3 Lagrange interpolation of E at the half-step position
+// (x - dt/2 vx, y - dt/2 vy) separates: the x part (base column ip0 and
+// weights cx) depends on the lane's (x, vx) only, the y part (row ip1 and
+// weights cy) on (y, vy), which is uniform in the workgroup at a step.  A lane
+// therefore keeps G[k1] = sum_k0 cx[k0] E(ip0+k0, ip1+k1) for the three rows
+// and, as ip1 advances by one per step, computes only the new row.
+template <typename T, typename A, int TVX> struct YStream {
+  static constexpr int VPL = 16 / (int)sizeof(T);  // slab values per lane load
+  const T *__restrict__ c4;
+  T *__restrict__ fout;
+  T *slab;      // [3][slab]
+  double *eb;   // [3][3][N0][2]
+  Geom g;
+  FeetConsts fc;
+  double dt;
+  int N0, N1, N2, N3, RX, slab_elems, nsteps, oyw;
+  int ix, tv, ivx, ivy;
+  bool active;
+  double x, vx, vy;
+  long S1, S2, S3;
+  int F0;
+  int wave, nwave, lane;
+  int force;  // bit 0: every lane reads rows per cell; bit 1: every foot by direct gather
+  int cbase;  // C4 element of (x -2, y -3, vx tile-2, vy ivy-2)
+  const unsigned *esrc;  // E DMA source of this lane: Ex or Ey dword of its point within a 16-point chunk
+  double cx[3];          // x Lagrange weights of the half-step foot
+  int jx[3];             // wrapped E columns ip0 .. ip0+2
+  double gx[3], gy[3];   // x-interpolated E of rows ip1 .. ip1+2
+  int ip1p;              // ip1 of the previous step
+  OpenCell<A> cell[4];
+  int bad;
+
+  // E row ip1 of the cell y (computeFeet: round of the half-step y) - 1.
+  __device__ __forceinline__ int erow0(int y) const {
+    const double yy = g.minPhy[1] + y * g.dx[1];
+    return (int)(round(half_y(g, yy, vy, dt)) - 1);
+  }
+
+  // Start moving step q's slab and E rows into LDS buffer b by LDS-DMA:
+  // slab rows (row = kvx + (TVX+4) kvy: vx tile-2+kvx, vy ivy-2+kvy) and
+  // 16-point chunks of the three E rows are dealt to the waves in turn.
+  // Returns the number of DMA instructions this wave issued (wave-uniform).
+  __device__ __forceinline__ int fetch(int q, int b) {
+    const int p = q + oyw - 1;  // y-position of the plane
+    const T *src = c4 + cbase + F0 * (p + 3);
+    T *sb = slab + b * slab_elems;
+    const int per_row = RX / VPL;
+    const long F01 = (long)F0 * (N1 + 6);
+    int n = 0;
+    for (int row = wave; row < 5 * (TVX + 4); row += nwave) {
+      const int kvx = row % (TVX + 4), kvy = row / (TVX + 4);
+      const T *rs = src + F01 * (kvx + (long)(N2 + 6) * kvy);
+      T *rl = sb + row * RX;
+      for (int c0 = 0; c0 < per_row; c0 += 64, n++)
+        if (c0 + lane < per_row)
+          __builtin_amdgcn_global_load_lds((const void *)(rs + VPL * (c0 + lane)), (lds_void *)(rl + VPL * c0), 16, 0,
+                                           0);
+    }
+    if (q < N1) {
+      const int ip1 = erow0(q);
+      const int cpr = N0 / 16;  // chunks per E row
+      unsigned *el = (unsigned *)(eb + b * 6 * N0);
+      for (int ch = wave; ch < 3 * cpr; ch += nwave, n++) {
+        const int k1 = ch / cpr, c = ch - k1 * cpr;
+        const int row = ((ip1 + k1) % N1 + N1) % N1;
+        __builtin_amdgcn_global_load_lds((const void *)(esrc + 2 * (row * N0 + 16 * c)), (lds_void *)(el + 64 * ch), 4,
+                                         0, 0);
+      }
+    }
+    return n;
+  }
+
+  // Per step, uniform in the workgroup: the y part of computeFeet for cell
+  // y = q, and the update of the lane's x-interpolated E rows (buffer b).
+  __device__ __forceinline__ void field_rows(int q, int b, double cy[3], double &yq) {
+    yq = g.minPhy[1] + q * g.dx[1];
+    const double xt1 = half_y(g, yq, vy, dt);
+    const int ip1 = (int)(round(xt1) - 1);
+    const double py = xt1 - ip1;
+    cy[0] = .5 * (py - 1.) * (py - 2.);
+    cy[1] = -1. * (py) * (py - 2.);
+    cy[2] = .5 * (py) * (py - 1.);
+    const double2 *er = (const double2 *)(eb + b * 6 * N0);
+    if (ip1 == ip1p + 1) {
+      gx[0] = gx[1];
+      gy[0] = gy[1];
+      gx[1] = gx[2];
+      gy[1] = gy[2];
+      const double2 e0 = er[2 * N0 + jx[0]], e1 = er[2 * N0 + jx[1]], e2 = er[2 * N0 + jx[2]];
+      gx[2] = cx[0] * e0.x + cx[1] * e1.x + cx[2] * e2.x;
+      gy[2] = cx[0] * e0.y + cx[1] * e1.y + cx[2] * e2.y;
+    } else {
+#pragma unroll
+      for (int k1 = 0; k1 < 3; k1++) {
+        const double2 e0 = er[k1 * N0 + jx[0]], e1 = er[k1 * N0 + jx[1]], e2 = er[k1 * N0 + jx[2]];
+        gx[k1] = cx[0] * e0.x + cx[1] * e1.x + cx[2] * e2.x;
+        gy[k1] = cx[0] * e0.y + cx[1] * e1.y + cx[2] * e2.y;
+      }
+    }
+    ip1p = ip1;
+  }
+
+  // Open cell y = q in slot R: the rest of computeFeet (VMO/Advection.hpp:
+  // 166-218), the bound check, B-spline weights and stencil offsets.
+  template <int R> __device__ __forceinline__ void open(int q, double yq, const double cy[3]) {
+    OpenCell<A> &c = cell[R];
+    c.acc = 0;
+    c.key = -1;
+    if (q >= N1) {
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) c.w[j][k] = 0;
+      return;
+    }
+    const double ftmp1 = cy[0] * gx[0] + cy[1] * gx[1] + cy[2] * gx[2];
+    const double ftmp2 = cy[0] * gy[0] + cy[1] * gy[1] + cy[2] * gy[2];
+    const double xt2 = vx - 0.5 * dt * ftmp1;
+    const double xt3 = vy - 0.5 * dt * ftmp2;
+    double xs[4];
+    xs[0] = x - dt * xt2;
+    xs[1] = yq - dt * xt3;
+    {
+      const double a = vx - dt * ftmp1, hi = g.minPhy[2] + fc.rxwidth[2], lo = g.minPhy[2];
+      const double m = (hi < a) ? hi : a;
+      xs[2] = (m < lo) ? lo : m;
+    }
+    {
+      const double a = vy - dt * ftmp2, hi = g.minPhy[3] + fc.rxwidth[3], lo = g.minPhy[3];
+      const double m = (hi < a) ? hi : a;
+      xs[3] = (m < lo) ? lo : m;
+    }
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 4; j++) ok = ok && (xs[j] >= fc.locmin[j] && xs[j] <= fc.locmax[j]);
+    int ip[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const double fj = g.inv_dx[j] * (xs[j] - g.minPhy[j]);
+      ip[j] = ok ? (int)floor(fj) : 0;
+      bspline_eta<A>((A)(fj - ip[j]), c.w[j]);
+    }
+    const int ox = ip[0] - ix, oy = ip[1] - q, ovx = ip[2] - ivx, ovy = ip[3] - ivy;
+    const bool fast = ok && (ox == -1 || ox == 0) && oy == oyw && (ovx == -1 || ovx == 0) &&
+                      (ovy == -1 || ovy == 0) && !(force & 2);
+    c.key = (ox + 1) | ((ovx + 1) << 1) | ((ovy + 1) << 2);
+    if (!fast) {
+      // out of domain (the reference stops with "Time step is too large"), or
+      // a foot outside the streamed pattern: direct gather, nothing streamed
+      if (ok) {
+        // a copy keeps the slot's weights out of the callee's reach (the
+        // per-lane state then stays in registers)
+        A e[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+          for (int k = 0; k < 4; k++) e[j][k] = c.w[j][k];
+        c.acc = gather4d<T, A>(c4, S1, S2, S3, ip, e);
+      }
+      else bad = 1;
+      c.key = -1;
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) c.w[j][k] = 0;
+    }
+  }
+
+  // LDS element of the first value of row (kvx = 0, kvy = 0) for a key.
+  __device__ __forceinline__ int row_base(int key) const {
+    const int ox = (key & 1) - 1, ovx = ((key >> 1) & 1) - 1, ovy = ((key >> 2) & 1) - 1;
+    return (tv + 1 + ovx) * RX + (1 + ovy) * (TVX + 4) * RX + ix + 1 + ox;
+  }
+
+  // Add plane q (LDS buffer b) to the four open cells; the cell opened at
+  // step q - j sits in slot (R - j) & 3 and takes y-weight j.
+  template <int R> __device__ __forceinline__ void accumulate(int b) {
+    const T *s = slab + b * slab_elems;
+    const int VS = (TVX + 4) * RX;  // vy row stride
+    int k0 = -1;
+    bool shared = !(force & 1);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int k = cell[c].key;
+      if (k >= 0) {
+        if (k0 < 0) k0 = k;
+        else shared = shared && (k == k0);
+      }
+    }
+    A t[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) t[c] = 0;
+    if (shared) {
+      const T *sp = s + row_base(k0 < 0 ? 0 : k0);
+#pragma unroll
+      for (int kvy = 0; kvy < 4; kvy++) {
+        A u[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) u[c] = 0;
+#pragma unroll
+        for (int kvx = 0; kvx < 4; kvx++) {
+          const T *rp = sp + kvy * VS + kvx * RX;
+          const A r0 = (A)rp[0], r1 = (A)rp[1], r2 = (A)rp[2], r3 = (A)rp[3];
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const A *w0 = cell[c].w[0];
+            const A d = w0[0] * r0 + w0[1] * r1 + w0[2] * r2 + w0[3] * r3;
+            u[c] += cell[c].w[2][kvx] * d;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) t[c] += cell[c].w[3][kvy] * u[c];
+      }
+    } else {
+      int bs[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) bs[c] = row_base(cell[c].key < 0 ? 0 : cell[c].key);
+#pragma unroll
+      for (int kvy = 0; kvy < 4; kvy++) {
+        A u[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) u[c] = 0;
+#pragma unroll
+        for (int kvx = 0; kvx < 4; kvx++) {
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const T *rp = s + bs[c] + kvy * VS + kvx * RX;
+            const A *w0 = cell[c].w[0];
+            const A d = w0[0] * (A)rp[0] + w0[1] * (A)rp[1] + w0[2] * (A)rp[2] + w0[3] * (A)rp[3];
+            u[c] += cell[c].w[2][kvx] * d;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) t[c] += cell[c].w[3][kvy] * u[c];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      OpenCell<A> &c = cell[(R - j) & 3];
+      c.acc += c.w[1][j] * t[(R - j) & 3];
+    }
+  }
+
+  // Write cell y = q - 3 (slot (R + 1) & 3).
+  template <int R> __device__ __forceinline__ void close(int q) {
+    const int y = q - 3;
+    if (y < 0 || y >= N1) return;
+    const A v = cell[(R + 1) & 3].acc;
+    fout[ix + (long)N0 * (y + (long)N1 * (ivx + (long)N2 * ivy))] = (T)v;
+  }
+
+  // Step q: plane q sits in LDS buffer b = q mod 3.  The loads of plane q+2
+  // are issued last; the wait before the barrier then leaves exactly those in
+  // flight (vector memory operations of a wave retire in issue order), so
+  // plane q+1 is complete in LDS for every wave after the barrier.
+  template <int R> __device__ __forceinline__ void step(int q, int b) {
+    if (active) {
+      double cy[3], yq;
+      field_rows(q, b, cy, yq);
+      open<R>(q, yq, cy);
+      accumulate<R>(b);
+      close<R>(q);
+    }
+    int n = 0;
+    if (q + 2 < nsteps) n = fetch(q + 2, b == 0 ? 2 : b - 1);
+    wait_vm(n);
+    __builtin_amdgcn_s_barrier();
+  }
+};
+
+}  // namespace
+
+template <typename T, typename A, int TVX>
+__global__ __launch_bounds__(512) void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ ex,
+                                                     const double *__restrict__ ey, Geom g, FeetConsts fc, double dt,
+                                                     T *__restrict__ fout, int *__restrict__ err, int force) {
+  extern __shared__ __align__(16) unsigned char ys_smem[];
+  YStream<T, A, TVX> s;
+  s.force = force;
+  s.c4 = c4;
+  s.fout = fout;
+  s.g = g;
+  s.fc = fc;
+  s.dt = dt;
+  s.N0 = g.n[0];
+  s.N1 = g.n[1];
+  s.N2 = g.n[2];
+  s.N3 = g.n[3];
+  s.RX = s.N0 + 4;
+  s.slab_elems = 5 * (TVX + 4) * s.RX;
+  s.slab = (T *)ys_smem;
+  s.eb = (double *)(ys_smem + 3 * (size_t)s.slab_elems * sizeof(T));
+  s.nsteps = s.N1 + 3;
+  const int ntile = s.N2 / TVX;
+  const int tile = blockIdx.x % ntile;
+  s.ivy = blockIdx.x / ntile;
+  const int tid = threadIdx.x;
+  s.active = tid < s.N0 * TVX;
+  s.ix = s.active ? tid % s.N0 : 0;
+  s.tv = s.active ? tid / s.N0 : 0;
+  s.ivx = tile * TVX + s.tv;
+  s.x = g.minPhy[0] + s.ix * g.dx[0];
+  s.vx = g.minPhy[2] + s.ivx * g.dx[2];
+  s.vy = g.minPhy[3] + s.ivy * g.dx[3];
+  s.oyw = s.vy > 0. ? -1 : 0;
+  s.F0 = s.N0 + 6;
+  s.S1 = s.N0 + 6;
+  s.S2 = s.S1 * (s.N1 + 6);
+  s.S3 = s.S2 * (s.N2 + 6);
+  s.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  s.lane = tid & 63;
+  s.nwave = (int)(blockDim.x >> 6);
+  // E DMA: lane l moves dword l % 2 of component (l % 4) / 2 of point l / 4
+  s.esrc = (const unsigned *)(((s.lane & 3) >> 1) ? ey : ex) + 2 * (s.lane >> 2) + (s.lane & 1);
+  {  // x part of computeFeet's E interpolation (VMO/Advection.hpp:174-196)
+    double xt0 = s.x - 0.5 * dt * s.vx;
+    xt0 = g.inv_dx[0] * (xt0 - g.minPhy[0]);
+    const int ip0 = (int)(round(xt0) - 1);
+    const double px = xt0 - ip0;
+    s.cx[0] = .5 * (px - 1.) * (px - 2.);
+    s.cx[1] = -1. * (px) * (px - 2.);
+    s.cx[2] = .5 * (px) * (px - 1.);
+#pragma unroll
+    for (int k0 = 0; k0 < 3; k0++) s.jx[k0] = ((ip0 + k0) % s.N0 + s.N0) % s.N0;
+  }
+  s.ip1p = -(1 << 30);
+  // position (x -2, y -3, vx tile-2, vy ivy-2) sits at storage (1, 0, tile+1, ivy+1)
+  s.cbase = 1 + s.F0 * (s.N1 + 6) * ((tile * TVX + 1) + (s.N2 + 6) * (s.ivy + 1));
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    s.cell[c].key = -1;
+    s.cell[c].acc = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) s.cell[c].w[j][k] = 0;
+  }
+  s.bad = 0;
+  // planes 0 and 1 in flight; wait for plane 0
+  s.fetch(0, 0);
+  const int n1 = s.nsteps > 1 ? s.fetch(1, 1) : 0;
+  wait_vm(n1);
+  __builtin_amdgcn_s_barrier();
+  int b = 0;  // q mod 3
+  for (int q0 = 0; q0 < s.nsteps; q0 += 4) {
+    s.template step<0>(q0, b);
+    b = b == 2 ? 0 : b + 1;
+    if (q0 + 1 < s.nsteps) s.template step<1>(q0 + 1, b);
+    b = b == 2 ? 0 : b + 1;
+    if (q0 + 2 < s.nsteps) s.template step<2>(q0 + 2, b);
+    b = b == 2 ? 0 : b + 1;
+    if (q0 + 3 < s.nsteps) s.template step<3>(q0 + 3, b);
+    b = b == 2 ? 0 : b + 1;
+  }
+  if (s.bad) atomicOr(err, 2);
+}
+
+namespace {
+
+struct YsPlan {
+  int tvx = 0;  // 0: not eligible
+  int block = 0;
+  size_t lds = 0;
+};
+
+YsPlan ys_plan(const Geom &g, size_t tsz) {
+  YsPlan p;
+  const long N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const long c4n = (N0 + 6) * (N1 + 6) * (N2 + 6) * (N3 + 6) + 64;
+  if (N0 % 2 || c4n >= (1L << 31)) return p;
+  int force = 0;
+  if (const char *e = getenv("VLP4D_YS_TVX")) force = atoi(e);
+  for (int tvx : {8, 4, 2}) {
+    if (force && tvx != force) continue;
+    if (N2 % tvx) continue;
+    const long lanes = N0 * tvx;
+    if (lanes > 512) continue;
+    const int block = (int)((lanes + 63) / 64 * 64);
+    const long slab = 5L * (tvx + 4) * (N0 + 4);
+    if ((N0 + 4) % (16 / (long)tsz) || N0 % 16) continue;  // whole 16-byte slab chunks, 16-point E chunks
+    const size_t lds = 3 * (size_t)slab * tsz + 3 * 6 * (size_t)N0 * sizeof(double);
+    if (lds > 160 * 1024) continue;
+    p.tvx = tvx;
+    p.block = block;
+    p.lds = lds;
+    return p;
+  }
+  return p;
+}
+
+template <typename T, typename A, int TVX>
+void launch_ys_t(const Geom &g, const YsPlan &p, const T *c4, const double *ex, const double *ey, double dt, T *fout,
+                 int *err, hipStream_t s) {
+  const void *fn = (const void *)k_advect4d_ys<T, A, TVX>;
+  if (p.lds > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
+  const unsigned grid = (unsigned)((g.n[2] / TVX) * g.n[3]);
+  int force = 0;  // test switch: VLP4D_YS_FORCE=1 rows per cell, 2 direct gather, 3 both
+  if (const char *e = getenv("VLP4D_YS_FORCE")) force = atoi(e) & 3;
+  hipLaunchKernelGGL((k_advect4d_ys<T, A, TVX>), dim3(grid), dim3(p.block), p.lds, s, c4, ex, ey, g,
+                     feet_consts(g), dt, fout, err, force);
+}
+
+}  // namespace
+
+// Returns false (nothing launched) when the geometry does not fit the
+// streaming kernel.
+template <typename T, typename A>
+bool launch_advect4d_ystream(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, T *fout,
+                             int *err, hipStream_t s) {
+  if (const char *e = getenv("VLP4D_YSTREAM"))
+    if (atoi(e) == 0) return false;
+  const YsPlan p = ys_plan(g, sizeof(T));
+  switch (p.tvx) {
+    case 8: launch_ys_t<T, A, 8>(g, p, c4, ex, ey, dt, fout, err, s); return true;
+    case 4: launch_ys_t<T, A, 4>(g, p, c4, ex, ey, dt, fout, err, s); return true;
+    case 2: launch_ys_t<T, A, 2>(g, p, c4, ex, ey, dt, fout, err, s); return true;
+    default: return false;
+  }
+}
+
+int ystream_tvx(const Geom &g, bool fp32_storage, bool arith32) {
+  if (const char *e = getenv("VLP4D_YSTREAM"))
+    if (atoi(e) == 0) return 0;
+  (void)arith32;
+  return ys_plan(g, fp32_storage ? sizeof(float) : sizeof(double)).tvx;
+}
+
+template bool launch_advect4d_ystream<double, double>(const Geom &, const double *, const double *, const double *,
+                                                      double, double *, int *, hipStream_t);
+template bool launch_advect4d_ystream<float, double>(const Geom &, const float *, const double *, const double *,
+                                                     double, float *, int *, hipStream_t);
+template bool launch_advect4d_ystream<float, float>(const Geom &, const float *, const double *, const double *,
+                                                    double, float *, int *, hipStream_t);
+
+}  // namespace vlp

@@ -51,6 +51,13 @@ void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, bool arith
 template <typename T>
 void launch_advect4d(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, bool arith32,
                      T *fout, double *rho_part, double *l2_part, int *err, hipStream_t s);
+// y-streaming 4-D advection (f only; rho and L2 by launch_moments afterwards).
+// Returns false without launching when the geometry does not fit it.
+template <typename T, typename A>
+bool launch_advect4d_ystream(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, T *fout,
+                             int *err, hipStream_t s);
+// vx tile of the y-streaming kernel for a geometry (0: not used).
+int ystream_tvx(const Geom &g, bool fp32_storage, bool arith32);
 template <typename T>
 void launch_moments(const Geom &g, const T *f, double *rho_part, double *l2_part, hipStream_t s);
 
@@ -62,7 +69,8 @@ long l2_partials(const Geom &g);
 // passes use the multi-lane scan kernel for all three dims, bit 2: 4-D
 // advection uses the vx-blocked kernel, bit 3: 4-cell blocks (else 2), bit 4:
 // E rows staged in LDS, bit 5: XCD-aware workgroup order, bit 6: float
-// advection arithmetic.
+// advection arithmetic, bit 7: the y-streaming 4-D advection replaces the
+// vx-blocked one, bits 8-11: its vx tile.
 int kernel_variants(const Geom &g, bool fp32_storage, bool arith32);
 
 // Field solve (FP64): rho from the per-vy partial sums, Poisson by DFT with

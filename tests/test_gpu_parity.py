@@ -124,13 +124,15 @@ def test_cfl_violation_reports_error(vlp):
         assert ei.value.code == vlp.E_CFL
 
 
-# The 4-D advection has tuned variants picked at launch from the environment
-# (cells per lane block, E rows staged in LDS when Nx % 64 == 0, XCD-aware
-# workgroup order).  Every combination must give the oracle's result; the
-# first shape takes the LDS path, the second cannot.
+# The vx-blocked 4-D advection (taken when the y-streaming kernel does not
+# fit the geometry, forced here with VLP4D_YSTREAM=0) has tuned variants picked
+# at launch from the environment (cells per lane block, E rows staged in LDS
+# when Nx % 64 == 0, XCD-aware workgroup order).  Every combination must give
+# the oracle's result; the first shape takes the LDS path, the second cannot.
 @pytest.mark.parametrize("n", [(64, 20, 24, 16), (40, 36, 48, 34)])
 @pytest.mark.parametrize("L,elds,remap", [(4, 1, 1), (4, 0, 0), (2, 1, 0), (2, 0, 1)])
 def test_advect4d_variants_match_oracle(vlp, orc, monkeypatch, n, L, elds, remap):
+    monkeypatch.setenv("VLP4D_YSTREAM", "0")
     monkeypatch.setenv("VLP4D_ADVECT_L", str(L))
     monkeypatch.setenv("VLP4D_ELDS", str(elds))
     monkeypatch.setenv("VLP4D_XCD_REMAP", str(remap))
@@ -149,3 +151,52 @@ def test_advect4d_variants_match_oracle(vlp, orc, monkeypatch, n, L, elds, remap
     assert bool(bits & 16) == (elds == 1 and n[0] % 64 == 0)
     assert relmax(f, st.f_interior()) < 1e-12
     assert relmax(d["nrj"], st.diags()["nrj"]) < 1e-12
+
+
+# The y-streaming 4-D advection: lanes whose four open cells agree on their
+# stencil offsets share one set of LDS rows, others read rows per cell, and
+# feet outside the streamed pattern (here: the vy = 0 slab, whose y offset
+# follows the sign of E) take the direct gather.  VLP4D_YS_FORCE makes every
+# lane take the per-cell rows (1), every foot the direct gather (2), or both
+# (3); each path must give the oracle's result.  Shapes: a vy = 0 slab (Nvy
+# even), vx tiles of 8 and 4, Nx a multiple of 16 but not of 64.
+@pytest.mark.parametrize("n", [(32, 12, 20, 14), (48, 20, 16, 18), (64, 16, 32, 12)])
+@pytest.mark.parametrize("force", [0, 1, 2, 3])
+def test_advect4d_ystream_paths_match_oracle(vlp, orc, monkeypatch, n, force):
+    monkeypatch.setenv("VLP4D_YS_FORCE", str(force))
+    steps = 2
+    dt = 0.5 * (4 * math.pi / n[0]) / 6.0
+    gc, oc = _configs(vlp, orc, n, dt=dt, nbiter=steps)
+    st = orc.State(oc)
+    st.step(steps)
+    with vlp.Vlp4d(gc) as sim:
+        bits = sim.kernel_variants()
+        sim.step(steps)
+        f = sim.get_f()
+        d = sim.diags()
+    assert bits & 128, "geometry expected to take the y-streaming kernel"
+    assert relmax(f, st.f_interior()) < 1e-12
+    for k in ("nrj", "nrjx", "nrjy", "l2norm"):
+        assert relmax(d[k], st.diags()[k]) < 1e-12, k
+
+
+@pytest.mark.parametrize("precision", ["FP32", "FP32_ACC"])
+def test_advect4d_ystream_fp32_matches_blocked(vlp, monkeypatch, precision):
+    """Float storage: the y-streaming kernel and the vx-blocked kernel compute
+    the same interpolation in the same arithmetic type; their results agree to
+    the rounding of that type (summation order differs)."""
+    n = (32, 16, 24, 14)
+    dt = 0.5 * (4 * math.pi / n[0]) / 6.0
+    gc = vlp.VlpConfig.make(*sld_box(n, vmax=6.0, dt=dt, nbiter=3))
+    out = {}
+    for ys in ("1", "0"):
+        monkeypatch.setenv("VLP4D_YSTREAM", ys)
+        with vlp.Vlp4d(gc, precision=getattr(vlp, precision)) as sim:
+            assert bool(sim.kernel_variants() & 128) == (ys == "1")
+            sim.step(3)
+            out[ys] = (sim.get_f(), sim.diags())
+    # a float ulp is 6e-8 relative; float sums in a different order differ by
+    # a few ulp per step
+    tol = 4e-7 if precision == "FP32" else 2e-6
+    assert relmax(out["1"][0], out["0"][0]) < tol
+    assert relmax(out["1"][1]["nrj"], out["0"][1]["nrj"]) < 1e-6
