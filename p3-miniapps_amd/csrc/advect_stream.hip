@@ -30,14 +30,30 @@ namespace vlp {
 
 namespace {
 
-// One open cell: weights [dim][k] (dims x, y, vx, vy), running sum, and the
-// packed stencil offsets (ox+1) | (ovx+1) << 1 | (ovy+1) << 2, or -1 when the
-// cell takes no streamed contribution.
+// One open cell: B-spline weights along x, y, vx, vy ([dim][k]; with double
+// arithmetic w[1] is unused and the y weight for the current plane is
+// evaluated from the fractional y position wy when needed), running sum, and
+// the packed stencil offsets
+// (ox+1) | (ovx+1) << 1 | (ovy+1) << 2, or -1 when the cell takes no streamed
+// contribution.
 template <typename A> struct OpenCell {
   A w[4][4];
+  A wy;
   A acc;
   int key;
 };
+
+// Weight j of the cubic B-spline at fractional position w: the same
+// expressions as bspline_eta (VMO/Advection.hpp:266-274), evaluated for one j.
+template <typename A, int J> __device__ __forceinline__ A bspline_eta_j(A w) {
+  const A e3 = (A)(1. / 6.) * w * w * w;
+  if (J == 3) return e3;
+  const A e0 = (A)(1. / 6.) + (A)0.5 * w * (w - (A)1.) - e3;
+  if (J == 0) return e0;
+  const A e2 = w + e0 - (A)2. * e3;
+  if (J == 2) return e2;
+  return (A)1. - e0 - e2 - e3;
+}
 
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -72,8 +88,11 @@ __device__ __forceinline__ void wait_vm(int n) {
 // weights cy) on (y, vy), which is uniform in the workgroup at a step.  A lane
 // therefore keeps G[k1] = sum_k0 cx[k0] E(ip0+k0, ip1+k1) for the three rows
 // and, as ip1 advances by one per step, computes only the new row.
-template <typename T, typename A, int TVX> struct YStream {
+template <typename T, typename A, int TVX, int NB> struct YStream {
   static constexpr int VPL = 16 / (int)sizeof(T);  // slab values per lane load
+  // double arithmetic keeps each cell's fractional y position instead of its
+  // four y weights (16 registers fewer per lane); float keeps the weights
+  static constexpr bool WY = sizeof(A) == sizeof(double);
   const T *__restrict__ c4;
   T *__restrict__ fout;
   T *slab;      // [3][slab]
@@ -84,6 +103,7 @@ template <typename T, typename A, int TVX> struct YStream {
   int N0, N1, N2, N3, RX, slab_elems, nsteps, oyw;
   int ix, tv, ivx, ivy;
   bool active;
+  bool wave_has_cells;  // some lane of this wave owns cells (wave-uniform)
   double x, vx, vy;
   long S1, S2, S3;
   int F0;
@@ -202,11 +222,20 @@ template <typename T, typename A, int TVX> struct YStream {
 #pragma unroll
     for (int j = 0; j < 4; j++) ok = ok && (xs[j] >= fc.locmin[j] && xs[j] <= fc.locmax[j]);
     int ip[4];
+    A wyk[4];  // y weights, kept only for a direct gather
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const double fj = g.inv_dx[j] * (xs[j] - g.minPhy[j]);
       ip[j] = ok ? (int)floor(fj) : 0;
-      bspline_eta<A>((A)(fj - ip[j]), c.w[j]);
+      if (j == 1) {
+        c.wy = (A)(fj - ip[j]);
+        bspline_eta<A>(c.wy, wyk);
+        if (!WY)
+#pragma unroll
+          for (int k = 0; k < 4; k++) c.w[1][k] = wyk[k];
+      } else {
+        bspline_eta<A>((A)(fj - ip[j]), c.w[j]);
+      }
     }
     const int ox = ip[0] - ix, oy = ip[1] - q, ovx = ip[2] - ivx, ovy = ip[3] - ivy;
     const bool fast = ok && (ox == -1 || ox == 0) && oy == oyw && (ovx == -1 || ovx == 0) &&
@@ -222,7 +251,7 @@ template <typename T, typename A, int TVX> struct YStream {
 #pragma unroll
         for (int j = 0; j < 4; j++)
 #pragma unroll
-          for (int k = 0; k < 4; k++) e[j][k] = c.w[j][k];
+          for (int k = 0; k < 4; k++) e[j][k] = j == 1 ? wyk[k] : c.w[j][k];
         c.acc = gather4d<T, A>(c4, S1, S2, S3, ip, e);
       }
       else bad = 1;
@@ -255,29 +284,43 @@ template <typename T, typename A, int TVX> struct YStream {
         else shared = shared && (k == k0);
       }
     }
-    A t[4];
+    A t[4], u[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++) t[c] = 0;
+    for (int c = 0; c < 4; c++) t[c] = u[c] = 0;
     if (shared) {
+      // rows (kvx, kvy) in order, the next row's LDS reads issued before the
+      // current row's FMAs (one row of look-ahead hides the LDS latency)
       const T *sp = s + row_base(k0 < 0 ? 0 : k0);
+      T r[2][4];
 #pragma unroll
-      for (int kvy = 0; kvy < 4; kvy++) {
-        A u[4];
+      for (int k = 0; k < 4; k++) r[0][k] = sp[k];
 #pragma unroll
-        for (int c = 0; c < 4; c++) u[c] = 0;
+      for (int i = 0; i < 16; i++) {
+        const int kvx = i & 3, kvy = i >> 2;
+        // row i complete (only LDS reads are waited on; the slab DMA stays in
+        // flight), then row i+1's reads go out ahead of row i's FMAs
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        if (i + 1 < 16) {
+          const T *rn = sp + ((i + 1) >> 2) * VS + ((i + 1) & 3) * RX;
 #pragma unroll
-        for (int kvx = 0; kvx < 4; kvx++) {
-          const T *rp = sp + kvy * VS + kvx * RX;
-          const A r0 = (A)rp[0], r1 = (A)rp[1], r2 = (A)rp[2], r3 = (A)rp[3];
+          for (int k = 0; k < 4; k++) r[(i + 1) & 1][k] = rn[k];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const A r0 = (A)r[i & 1][0], r1 = (A)r[i & 1][1], r2 = (A)r[i & 1][2], r3 = (A)r[i & 1][3];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const A *w0 = cell[c].w[0];
+          const A d = w0[0] * r0 + w0[1] * r1 + w0[2] * r2 + w0[3] * r3;
+          u[c] += cell[c].w[2][kvx] * d;
+        }
+        if (kvx == 3) {
 #pragma unroll
           for (int c = 0; c < 4; c++) {
-            const A *w0 = cell[c].w[0];
-            const A d = w0[0] * r0 + w0[1] * r1 + w0[2] * r2 + w0[3] * r3;
-            u[c] += cell[c].w[2][kvx] * d;
+            t[c] += cell[c].w[3][kvy] * u[c];
+            u[c] = 0;
           }
         }
-#pragma unroll
-        for (int c = 0; c < 4; c++) t[c] += cell[c].w[3][kvy] * u[c];
+        __builtin_amdgcn_sched_barrier(0);
       }
     } else {
       int bs[4];
@@ -285,9 +328,6 @@ template <typename T, typename A, int TVX> struct YStream {
       for (int c = 0; c < 4; c++) bs[c] = row_base(cell[c].key < 0 ? 0 : cell[c].key);
 #pragma unroll
       for (int kvy = 0; kvy < 4; kvy++) {
-        A u[4];
-#pragma unroll
-        for (int c = 0; c < 4; c++) u[c] = 0;
 #pragma unroll
         for (int kvx = 0; kvx < 4; kvx++) {
 #pragma unroll
@@ -299,14 +339,23 @@ template <typename T, typename A, int TVX> struct YStream {
           }
         }
 #pragma unroll
-        for (int c = 0; c < 4; c++) t[c] += cell[c].w[3][kvy] * u[c];
+        for (int c = 0; c < 4; c++) {
+          t[c] += cell[c].w[3][kvy] * u[c];
+          u[c] = 0;
+        }
       }
     }
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      OpenCell<A> &c = cell[(R - j) & 3];
-      c.acc += c.w[1][j] * t[(R - j) & 3];
-    }
+    close_plane<R, 0>(t);
+    close_plane<R, 1>(t);
+    close_plane<R, 2>(t);
+    close_plane<R, 3>(t);
+  }
+
+  // The cell opened J steps ago takes the plane's contribution with its y
+  // weight J.
+  template <int R, int J> __device__ __forceinline__ void close_plane(const A t[4]) {
+    OpenCell<A> &c = cell[(R - J) & 3];
+    c.acc += (WY ? bspline_eta_j<A, J>(c.wy) : c.w[1][J]) * t[(R - J) & 3];
   }
 
   // Write cell y = q - 3 (slot (R + 1) & 3).
@@ -317,11 +366,16 @@ template <typename T, typename A, int TVX> struct YStream {
     fout[ix + (long)N0 * (y + (long)N1 * (ivx + (long)N2 * ivy))] = (T)v;
   }
 
-  // Step q: plane q sits in LDS buffer b = q mod 3.  The loads of plane q+2
-  // are issued last; the wait before the barrier then leaves exactly those in
-  // flight (vector memory operations of a wave retire in issue order), so
-  // plane q+1 is complete in LDS for every wave after the barrier.
+  // Step q: plane q sits in LDS buffer b = q mod NB; vector memory operations
+  // of a wave retire in issue order, so a vmcnt wait on all but the newest
+  // operations completes every older load.
+  //  NB = 3: the loads of plane q+2 are issued last; the wait before the
+  //   barrier leaves exactly those in flight, so plane q+1 is complete in LDS
+  //   for every wave after the barrier.
+  //  NB = 2: the loads of plane q+1 are issued first and overlap the step;
+  //   the wait leaves only this step's store of f in flight.
   template <int R> __device__ __forceinline__ void step(int q, int b) {
+    if (NB == 2 && q + 1 < nsteps) fetch(q + 1, b ^ 1);
     if (active) {
       double cy[3], yq;
       field_rows(q, b, cy, yq);
@@ -330,7 +384,11 @@ template <typename T, typename A, int TVX> struct YStream {
       close<R>(q);
     }
     int n = 0;
-    if (q + 2 < nsteps) n = fetch(q + 2, b == 0 ? 2 : b - 1);
+    if (NB == 3) {
+      if (q + 2 < nsteps) n = fetch(q + 2, b == 0 ? 2 : b - 1);
+    } else {
+      n = (wave_has_cells && q >= 3 && q - 3 < N1) ? 1 : 0;
+    }
     wait_vm(n);
     __builtin_amdgcn_s_barrier();
   }
@@ -338,12 +396,12 @@ template <typename T, typename A, int TVX> struct YStream {
 
 }  // namespace
 
-template <typename T, typename A, int TVX>
+template <typename T, typename A, int TVX, int NB>
 __global__ __launch_bounds__(512) void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ ex,
                                                      const double *__restrict__ ey, Geom g, FeetConsts fc, double dt,
                                                      T *__restrict__ fout, int *__restrict__ err, int force) {
   extern __shared__ __align__(16) unsigned char ys_smem[];
-  YStream<T, A, TVX> s;
+  YStream<T, A, TVX, NB> s;
   s.force = force;
   s.c4 = c4;
   s.fout = fout;
@@ -357,7 +415,7 @@ __global__ __launch_bounds__(512) void k_advect4d_ys(const T *__restrict__ c4, c
   s.RX = s.N0 + 4;
   s.slab_elems = 5 * (TVX + 4) * s.RX;
   s.slab = (T *)ys_smem;
-  s.eb = (double *)(ys_smem + 3 * (size_t)s.slab_elems * sizeof(T));
+  s.eb = (double *)(ys_smem + NB * (size_t)s.slab_elems * sizeof(T));
   s.nsteps = s.N1 + 3;
   const int ntile = s.N2 / TVX;
   const int tile = blockIdx.x % ntile;
@@ -398,27 +456,32 @@ __global__ __launch_bounds__(512) void k_advect4d_ys(const T *__restrict__ c4, c
   for (int c = 0; c < 4; c++) {
     s.cell[c].key = -1;
     s.cell[c].acc = 0;
+    s.cell[c].wy = 0;
 #pragma unroll
     for (int j = 0; j < 4; j++)
 #pragma unroll
       for (int k = 0; k < 4; k++) s.cell[c].w[j][k] = 0;
   }
   s.bad = 0;
-  // planes 0 and 1 in flight; wait for plane 0
-  s.fetch(0, 0);
-  const int n1 = s.nsteps > 1 ? s.fetch(1, 1) : 0;
-  wait_vm(n1);
+  s.wave_has_cells = s.wave * 64 < s.N0 * TVX;
+  if (NB == 3) {  // planes 0 and 1 in flight; wait for plane 0
+    s.fetch(0, 0);
+    wait_vm(s.fetch(1, 1));
+  } else {
+    s.fetch(0, 0);
+    wait_vm(0);
+  }
   __builtin_amdgcn_s_barrier();
-  int b = 0;  // q mod 3
+  int b = 0;  // q mod NB
   for (int q0 = 0; q0 < s.nsteps; q0 += 4) {
     s.template step<0>(q0, b);
-    b = b == 2 ? 0 : b + 1;
+    b = b == NB - 1 ? 0 : b + 1;
     if (q0 + 1 < s.nsteps) s.template step<1>(q0 + 1, b);
-    b = b == 2 ? 0 : b + 1;
+    b = b == NB - 1 ? 0 : b + 1;
     if (q0 + 2 < s.nsteps) s.template step<2>(q0 + 2, b);
-    b = b == 2 ? 0 : b + 1;
+    b = b == NB - 1 ? 0 : b + 1;
     if (q0 + 3 < s.nsteps) s.template step<3>(q0 + 3, b);
-    b = b == 2 ? 0 : b + 1;
+    b = b == NB - 1 ? 0 : b + 1;
   }
   if (s.bad) atomicOr(err, 2);
 }
@@ -427,6 +490,7 @@ namespace {
 
 struct YsPlan {
   int tvx = 0;  // 0: not eligible
+  int nb = 3;   // LDS plane buffers
   int block = 0;
   size_t lds = 0;
 };
@@ -436,19 +500,29 @@ YsPlan ys_plan(const Geom &g, size_t tsz) {
   const long N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
   const long c4n = (N0 + 6) * (N1 + 6) * (N2 + 6) * (N3 + 6) + 64;
   if (N0 % 2 || c4n >= (1L << 31)) return p;
-  int force = 0;
+  int force = 0, force_nb = 0;
   if (const char *e = getenv("VLP4D_YS_TVX")) force = atoi(e);
-  for (int tvx : {8, 4, 2}) {
+  if (const char *e = getenv("VLP4D_YS_NBUF")) force_nb = atoi(e);
+  // Preference order measured at 128^4: float slabs run best as two or three
+  // small workgroups per CU (vx tile 2, two buffers: independent barriers,
+  // three waves per SIMD for float arithmetic); double slabs as one
+  // workgroup of vx tile 4 with two buffers.
+  static const int pref_f[][2] = {{2, 2}, {4, 2}, {8, 2}, {4, 3}, {2, 3}, {8, 3}};
+  static const int pref_d[][2] = {{4, 2}, {8, 2}, {2, 2}, {4, 3}, {8, 3}, {2, 3}};
+  for (const auto &c : (tsz == sizeof(float) ? pref_f : pref_d)) {
+    const int tvx = c[0], nb = c[1];
     if (force && tvx != force) continue;
+    if (force_nb && nb != force_nb) continue;
     if (N2 % tvx) continue;
     const long lanes = N0 * tvx;
     if (lanes > 512) continue;
     const int block = (int)((lanes + 63) / 64 * 64);
     const long slab = 5L * (tvx + 4) * (N0 + 4);
     if ((N0 + 4) % (16 / (long)tsz) || N0 % 16) continue;  // whole 16-byte slab chunks, 16-point E chunks
-    const size_t lds = 3 * (size_t)slab * tsz + 3 * 6 * (size_t)N0 * sizeof(double);
+    const size_t lds = nb * (size_t)slab * tsz + nb * 6 * (size_t)N0 * sizeof(double);
     if (lds > 160 * 1024) continue;
     p.tvx = tvx;
+    p.nb = nb;
     p.block = block;
     p.lds = lds;
     return p;
@@ -456,16 +530,23 @@ YsPlan ys_plan(const Geom &g, size_t tsz) {
   return p;
 }
 
-template <typename T, typename A, int TVX>
+template <typename T, typename A, int TVX, int NB>
 void launch_ys_t(const Geom &g, const YsPlan &p, const T *c4, const double *ex, const double *ey, double dt, T *fout,
                  int *err, hipStream_t s) {
-  const void *fn = (const void *)k_advect4d_ys<T, A, TVX>;
+  const void *fn = (const void *)k_advect4d_ys<T, A, TVX, NB>;
   if (p.lds > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
   const unsigned grid = (unsigned)((g.n[2] / TVX) * g.n[3]);
   int force = 0;  // test switch: VLP4D_YS_FORCE=1 rows per cell, 2 direct gather, 3 both
   if (const char *e = getenv("VLP4D_YS_FORCE")) force = atoi(e) & 3;
-  hipLaunchKernelGGL((k_advect4d_ys<T, A, TVX>), dim3(grid), dim3(p.block), p.lds, s, c4, ex, ey, g,
+  hipLaunchKernelGGL((k_advect4d_ys<T, A, TVX, NB>), dim3(grid), dim3(p.block), p.lds, s, c4, ex, ey, g,
                      feet_consts(g), dt, fout, err, force);
+}
+
+template <typename T, typename A, int TVX>
+void launch_ys_nb(const Geom &g, const YsPlan &p, const T *c4, const double *ex, const double *ey, double dt,
+                  T *fout, int *err, hipStream_t s) {
+  if (p.nb == 2) launch_ys_t<T, A, TVX, 2>(g, p, c4, ex, ey, dt, fout, err, s);
+  else launch_ys_t<T, A, TVX, 3>(g, p, c4, ex, ey, dt, fout, err, s);
 }
 
 }  // namespace
@@ -479,9 +560,9 @@ bool launch_advect4d_ystream(const Geom &g, const T *c4, const double *ex, const
     if (atoi(e) == 0) return false;
   const YsPlan p = ys_plan(g, sizeof(T));
   switch (p.tvx) {
-    case 8: launch_ys_t<T, A, 8>(g, p, c4, ex, ey, dt, fout, err, s); return true;
-    case 4: launch_ys_t<T, A, 4>(g, p, c4, ex, ey, dt, fout, err, s); return true;
-    case 2: launch_ys_t<T, A, 2>(g, p, c4, ex, ey, dt, fout, err, s); return true;
+    case 8: launch_ys_nb<T, A, 8>(g, p, c4, ex, ey, dt, fout, err, s); return true;
+    case 4: launch_ys_nb<T, A, 4>(g, p, c4, ex, ey, dt, fout, err, s); return true;
+    case 2: launch_ys_nb<T, A, 2>(g, p, c4, ex, ey, dt, fout, err, s); return true;
     default: return false;
   }
 }

@@ -158,12 +158,16 @@ def test_advect4d_variants_match_oracle(vlp, orc, monkeypatch, n, L, elds, remap
 # feet outside the streamed pattern (here: the vy = 0 slab, whose y offset
 # follows the sign of E) take the direct gather.  VLP4D_YS_FORCE makes every
 # lane take the per-cell rows (1), every foot the direct gather (2), or both
-# (3); each path must give the oracle's result.  Shapes: a vy = 0 slab (Nvy
-# even), vx tiles of 8 and 4, Nx a multiple of 16 but not of 64.
+# (3); each path must give the oracle's result, with two LDS plane buffers
+# (the next plane's loads overlap the step) and with three (loads two steps
+# ahead).  Shapes: a vy = 0 slab (Nvy even), vx tiles of 8 and 4, Nx a
+# multiple of 16 but not of 64.
 @pytest.mark.parametrize("n", [(32, 12, 20, 14), (48, 20, 16, 18), (64, 16, 32, 12)])
 @pytest.mark.parametrize("force", [0, 1, 2, 3])
-def test_advect4d_ystream_paths_match_oracle(vlp, orc, monkeypatch, n, force):
+@pytest.mark.parametrize("nbuf", [2, 3])
+def test_advect4d_ystream_paths_match_oracle(vlp, orc, monkeypatch, n, force, nbuf):
     monkeypatch.setenv("VLP4D_YS_FORCE", str(force))
+    monkeypatch.setenv("VLP4D_YS_NBUF", str(nbuf))
     steps = 2
     dt = 0.5 * (4 * math.pi / n[0]) / 6.0
     gc, oc = _configs(vlp, orc, n, dt=dt, nbiter=steps)
