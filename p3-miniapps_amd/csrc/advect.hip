@@ -452,8 +452,9 @@ int kernel_variants(const Geom &g, bool fp32_storage, bool arith32) {
   const bool remap = t.remap && (l2_partials(g) % 8 == 0);
   const int L = advect_L();
   const int ys = ystream_tvx(g, fp32_storage, arith32);
+  const bool xy = xy_fused_groups(g, fp32_storage ? sizeof(float) : sizeof(double)) != 0;
   return spline_variant_bits(g) | 4 | (L == 4 ? 8 : 0) | (use_elds(g) ? 16 : 0) | (remap ? 32 : 0) |
-         (arith32 ? 64 : 0) | (ys ? 128 | (ys << 8) : 0);
+         (arith32 ? 64 : 0) | (ys ? 128 | (ys << 8) : 0) | (xy ? 4096 : 0);
 }
 
 #define VLP_INST(T)                                                                                        \

@@ -4,22 +4,23 @@
 namespace vlp {
 
 // ---------------------------------------------------------------------------
-// Field solve, FP64.  rho = dvx*dvy*sum_vy rho_part (VMO/Field.cpp:37-53),
+// Field solve, FP64.  rho = dvx*dvy*sum rho_part over its outer planes
+// (vy, and the vx groups of the fused xy kernel; VMO/Field.cpp:37-53),
 // rho -= 1 for idcase 10/20 (VMO/Field.cpp:110-126), then solve_poisson_fftw
 // (VMO/Efield.cpp:54-106) with the FFT conventions of lib/OpenMP_FFT.hpp:
 // unnormalised r2c along x, c2c along y, normalisation in k-space, inverse
 // c2c along y, c2r along x (imaginary parts of DC/Nyquist ignored).  The
 // transforms are direct DFTs with exactly reduced twiddle tables.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_rho_rfftx(const double *__restrict__ rho_part, Geom g, FieldBufs b,
-                                                   int sub1) {
+__global__ __launch_bounds__(256) void k_rho_rfftx(const double *__restrict__ rho_part, long rho_outer, Geom g,
+                                                   FieldBufs b, int sub1) {
   extern __shared__ double row[];
-  const int N0 = g.n[0], N1 = g.n[1], N3 = g.n[3];
+  const int N0 = g.n[0], N1 = g.n[1];
   const int iy = blockIdx.x;
   const int nxh = N0 / 2 + 1;
   for (int ix = threadIdx.x; ix < N0; ix += blockDim.x) {
     double s = 0.;
-    for (int ivy = 0; ivy < N3; ivy++) s += rho_part[ix + (long)N0 * (iy + (long)N1 * ivy)];
+    for (long o = 0; o < rho_outer; o++) s += rho_part[ix + (long)N0 * (iy + (long)N1 * o)];
     double r = s * g.dx[2] * g.dx[3];
     if (sub1) r -= 1.;
     row[ix] = r;
@@ -165,14 +166,15 @@ __global__ __launch_bounds__(256) void k_c2rx_diag(Geom g, FieldBufs b) {
 }
 
 // Analytic fields of idcase 2 and 6 (VMO/Field.cpp:81-108); rho is kept.
-__global__ __launch_bounds__(256) void k_analytic_field(const double *__restrict__ rho_part, Geom g, FieldBufs b) {
+__global__ __launch_bounds__(256) void k_analytic_field(const double *__restrict__ rho_part, long rho_outer, Geom g,
+                                                        FieldBufs b) {
   __shared__ double red[8];
-  const int N0 = g.n[0], N1 = g.n[1], N3 = g.n[3];
+  const int N0 = g.n[0], N1 = g.n[1];
   const int iy = blockIdx.x;
   double pm = 0., pe = 0., pex = 0., pey = 0.;
   for (int ix = threadIdx.x; ix < N0; ix += blockDim.x) {
     double s = 0.;
-    for (int ivy = 0; ivy < N3; ivy++) s += rho_part[ix + (long)N0 * (iy + (long)N1 * ivy)];
+    for (long o = 0; o < rho_outer; o++) s += rho_part[ix + (long)N0 * (iy + (long)N1 * o)];
     const double r = s * g.dx[2] * g.dx[3];
     const double e = (g.idcase == 2) ? -(g.minPhy[0] + ix * g.dx[0]) : -(g.minPhy[1] + iy * g.dx[1]);
     const long id = ix + (long)N0 * iy;
@@ -224,15 +226,15 @@ __global__ __launch_bounds__(256) void k_diag_final(Geom g, FieldBufs b, const d
   }
 }
 
-void launch_field(const Geom &g, const FieldBufs &b, const double *rho_part, const double *l2_part, long n_l2, int iter,
-                  bool record_diag, hipStream_t s) {
+void launch_field(const Geom &g, const FieldBufs &b, const double *rho_part, long rho_outer, const double *l2_part,
+                  long n_l2, int iter, bool record_diag, hipStream_t s) {
   const int N0 = g.n[0], N1 = g.n[1];
   const int nxh = N0 / 2 + 1;
   if (g.idcase == 2 || g.idcase == 6) {
-    hipLaunchKernelGGL(k_analytic_field, dim3(N1), dim3(256), 0, s, rho_part, g, b);
+    hipLaunchKernelGGL(k_analytic_field, dim3(N1), dim3(256), 0, s, rho_part, rho_outer, g, b);
   } else {
     const int sub1 = (g.idcase == 10 || g.idcase == 20);
-    hipLaunchKernelGGL(k_rho_rfftx, dim3(N1), dim3(256), sizeof(double) * N0, s, rho_part, g, b, sub1);
+    hipLaunchKernelGGL(k_rho_rfftx, dim3(N1), dim3(256), sizeof(double) * N0, s, rho_part, rho_outer, g, b, sub1);
     hipLaunchKernelGGL(k_poisson_y, dim3(nxh), dim3(256), sizeof(double) * 8 * N1, s, g, b);
     hipLaunchKernelGGL(k_c2rx_diag, dim3(N1), dim3(256), sizeof(double) * (6 * nxh + 8), s, g, b);
   }

@@ -41,6 +41,9 @@ struct vlp_ctx {
   size_t n_c4s = 0;  // stored C4: positions -3..N+2 per dim plus a zeroed tail
   double *rho_part = nullptr, *l2_part = nullptr;
   long n_l2 = 0;
+  int xy_groups = 0;        // G of the fused xy kernel (0: separate kernels)
+  long rho_outer = 0;       // density-partial planes currently in rho_part
+  bool half_rho = false;    // rho_part holds the half-step density of the last spline_xy
   FieldBufs fb{};
   int *d_err = nullptr;
   int *h_err = nullptr;
@@ -136,6 +139,26 @@ void free_all(vlp_ctx *c) {
 
 template <typename T> int run_spline_xy(vlp_ctx *c) {
   const Geom &g = c->g;
+  c->half_rho = false;
+  if (c->xy_groups) {
+    // fused: CXY and the density of the 2-D half step in one pass
+    bool ok;
+    if constexpr (sizeof(T) == sizeof(float)) {
+      ok = c->precision == VLP_FP32_ACC
+               ? vlp::launch_xy_adv2d<T, float>(g, (const T *)c->f, (T *)c->cxy, 0.5 * c->cfg.dt, c->rho_part,
+                                                c->d_err, c->stream)
+               : vlp::launch_xy_adv2d<T, double>(g, (const T *)c->f, (T *)c->cxy, 0.5 * c->cfg.dt, c->rho_part,
+                                                 c->d_err, c->stream);
+    } else {
+      ok = vlp::launch_xy_adv2d<T, double>(g, (const T *)c->f, (T *)c->cxy, 0.5 * c->cfg.dt, c->rho_part, c->d_err,
+                                           c->stream);
+    }
+    if (ok) {
+      c->half_rho = true;
+      c->rho_outer = (long)c->xy_groups * g.n[3];
+      return VLP_OK;
+    }
+  }
   const long N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
   const long E0 = g.n[0] + 4, E1 = N1 + 4;
   T *tx = (T *)c->tv;
@@ -175,17 +198,23 @@ template <typename T> int run_spline_vxvy(vlp_ctx *c) {
 }
 
 template <typename T> void run_advect2d(vlp_ctx *c) {
+  if (c->half_rho) return;  // the fused xy kernel produced it
   vlp::launch_advect2d_rho<T>(c->g, (const T *)c->cxy, 0.5 * c->cfg.dt, c->precision == VLP_FP32_ACC, c->rho_part,
                               c->d_err, c->stream);
+  c->rho_outer = c->g.n[3];
 }
 
 template <typename T> void run_advect4d(vlp_ctx *c) {
   vlp::launch_advect4d<T>(c->g, (const T *)c->c4, c->fb.ex, c->fb.ey, c->cfg.dt, c->precision == VLP_FP32_ACC,
                           (T *)c->fnext, c->rho_part, c->l2_part, c->d_err, c->stream);
+  c->rho_outer = c->g.n[3];
+  c->half_rho = false;
 }
 
 template <typename T> void run_moments(vlp_ctx *c) {
   vlp::launch_moments<T>(c->g, (const T *)c->f, c->rho_part, c->l2_part, c->stream);
+  c->rho_outer = c->g.n[3];
+  c->half_rho = false;
 }
 
 int check_launch(vlp_ctx *c, const char *what) {
@@ -209,7 +238,8 @@ int sync_and_check(vlp_ctx *c) {
 }
 
 int field(vlp_ctx *c, int iter, bool record, bool with_l2) {
-  vlp::launch_field(c->g, c->fb, c->rho_part, with_l2 ? c->l2_part : nullptr, c->n_l2, iter, record, c->stream);
+  vlp::launch_field(c->g, c->fb, c->rho_part, c->rho_outer, with_l2 ? c->l2_part : nullptr, c->n_l2, iter, record,
+                    c->stream);
   return check_launch(c, "field");
 }
 
@@ -381,6 +411,8 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
   c->n_c4 = (N0 + 4) * (N1 + 4) * (N2 + 4) * (N3 + 4);
   c->n_c4s = (N0 + 6) * (N1 + 6) * (N2 + 6) * (N3 + 6) + 64;
   c->n_l2 = vlp::l2_partials(c->g);
+  c->xy_groups = vlp::xy_fused_groups(c->g, c->esz);
+  c->rho_outer = N3;
   const size_t nxh = N0 / 2 + 1;
   auto fail = [&](int code) {
     free_all(c);
@@ -402,7 +434,7 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
   ALLOC(c->cxy, c->n_cxy * c->esz);
   ALLOC(c->tv, c->n_tv * c->esz);
   ALLOC(c->c4, c->n_c4s * c->esz);
-  ALLOC(c->rho_part, N0 * N1 * N3 * sizeof(double));
+  ALLOC(c->rho_part, N0 * N1 * N3 * (c->xy_groups > 1 ? (size_t)c->xy_groups : 1) * sizeof(double));
   ALLOC(c->l2_part, (size_t)c->n_l2 * sizeof(double));
   ALLOC(c->fb.rho, N0 * N1 * sizeof(double));
   ALLOC(c->fb.ex, N0 * N1 * sizeof(double));

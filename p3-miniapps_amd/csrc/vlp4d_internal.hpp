@@ -51,6 +51,14 @@ void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, bool arith
 template <typename T>
 void launch_advect4d(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, bool arith32,
                      T *fout, double *rho_part, double *l2_part, int *err, hipStream_t s);
+// Fused spline x/y + 2-D half-step advection + density partials (xy_fused.hip):
+// f -> CXY and rho_part[grp][vy][y][x] for grp < G.  Returns false without
+// launching when the geometry does not fit; xy_fused_groups gives G (0: the
+// separate x, y and advect2d kernels run instead).
+template <typename T, typename A>
+bool launch_xy_adv2d(const Geom &g, const T *f, T *cxy, double dt_half, double *rho_part, int *err, hipStream_t s);
+int xy_fused_groups(const Geom &g, size_t tsz);
+
 // y-streaming 4-D advection (f only; rho and L2 by launch_moments afterwards).
 // Returns false without launching when the geometry does not fit it.
 template <typename T, typename A>
@@ -70,7 +78,8 @@ long l2_partials(const Geom &g);
 // advection uses the vx-blocked kernel, bit 3: 4-cell blocks (else 2), bit 4:
 // E rows staged in LDS, bit 5: XCD-aware workgroup order, bit 6: float
 // advection arithmetic, bit 7: the y-streaming 4-D advection replaces the
-// vx-blocked one, bits 8-11: its vx tile.
+// vx-blocked one, bits 8-11: its vx tile, bit 12: the fused xy spline + 2-D
+// advection kernel replaces the x, y and advect2d kernels.
 int kernel_variants(const Geom &g, bool fp32_storage, bool arith32);
 
 // Field solve (FP64): rho from the per-vy partial sums, Poisson by DFT with
@@ -85,7 +94,9 @@ struct FieldBufs {
   double *diag;       // 5*(nbiter+1): nrj, nrjx, nrjy, mass, l2
   int nbiter;
 };
-void launch_field(const Geom &g, const FieldBufs &b, const double *rho_part, const double *l2_part, long n_l2,
-                  int iter, bool record_diag, hipStream_t s);
+// rho_outer: number of N0*N1 density-partial planes in rho_part (N3, or
+// G*N3 after the fused xy kernel).
+void launch_field(const Geom &g, const FieldBufs &b, const double *rho_part, long rho_outer, const double *l2_part,
+                  long n_l2, int iter, bool record_diag, hipStream_t s);
 
 }  // namespace vlp

@@ -204,3 +204,54 @@ def test_advect4d_ystream_fp32_matches_blocked(vlp, monkeypatch, precision):
     tol = 4e-7 if precision == "FP32" else 2e-6
     assert relmax(out["1"][0], out["0"][0]) < tol
     assert relmax(out["1"][1]["nrj"], out["0"][1]["nrj"]) < 1e-6
+
+
+# The fused x/y spline + 2-D advection kernel (one (vx, vy) plane per LDS
+# pass, density of the half step summed over groups of vx planes) against the
+# oracle's computeCoeff_xy, advect_2D_xy and field solve, then whole steps.
+# Shapes take both chunk sizes (C = 12: Nx = 48, Ny = 24; C = 16) and uneven
+# lane counts per line; VLP4D_XY_FUSED=0 must give the same result through the
+# separate kernels.
+XY_SHAPES = [(48, 24, 20, 14), (32, 16, 24, 18), (64, 32, 16, 12)]
+
+
+@pytest.mark.parametrize("n", XY_SHAPES)
+def test_xy_fused_phases_match_oracle(vlp, orc, n):
+    gc, oc = _configs(vlp, orc, n)
+    rng = np.random.default_rng(4321)
+    h = orc.init_case(oc)
+    orc.interior_of(h)[...] *= 1.0 + 0.1 * rng.standard_normal(orc.interior_shape(oc))
+    f0 = np.ascontiguousarray(orc.interior_of(h))
+    with vlp.Vlp4d(gc) as sim:
+        assert sim.kernel_variants() & 4096, "geometry expected to take the fused xy kernel"
+        sim.set_f(f0)
+        orc.exchange_halo(oc, h)
+        orc.spline_xy(oc, h)
+        sim.spline_xy()
+        assert relmax(sim.buffer(vlp.BUF_CXY), _cxy_view(orc, h)) < 1e-13
+        assert orc.advect_2d(oc, h, 0.5 * oc.dt) == 0
+        rho = orc.field_rho(oc, h)
+        phi, ex, ey = orc.field_poisson(oc, rho)
+        sim.advect2d_poisson()
+        assert relmax(sim.buffer(vlp.BUF_EX), ex) < 1e-12
+        assert relmax(sim.buffer(vlp.BUF_EY), ey) < 1e-12
+        assert relmax(sim.buffer(vlp.BUF_RHO), phi) < 1e-12
+
+
+@pytest.mark.parametrize("n", XY_SHAPES)
+@pytest.mark.parametrize("fused", ["1", "0"])
+def test_xy_fused_multistep_matches_oracle(vlp, orc, monkeypatch, n, fused):
+    monkeypatch.setenv("VLP4D_XY_FUSED", fused)
+    steps = 3
+    dt = 0.5 * (4 * math.pi / n[0]) / 6.0
+    gc, oc = _configs(vlp, orc, n, dt=dt, nbiter=steps)
+    st = orc.State(oc)
+    st.step(steps)
+    with vlp.Vlp4d(gc) as sim:
+        assert bool(sim.kernel_variants() & 4096) == (fused == "1")
+        sim.step(steps)
+        f = sim.get_f()
+        d = sim.diags()
+    assert relmax(f, st.f_interior()) < 1e-12
+    for k in ("nrj", "nrjx", "nrjy", "l2norm"):
+        assert relmax(d[k], st.diags()[k]) < 1e-12, k
