@@ -90,16 +90,17 @@ def advect4d_bytes(n, b):
     return b * ((n + 4) ** 4 + n ** 4)
 
 
-def measured_traffic(precision, n):
-    """HBM bytes per k_advect4d_blk launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
-    or None when no summary for this precision and size exists."""
+def measured_traffic(kernel, precision, n):
+    """HBM bytes per launch of the 4-D advection kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration), or None when no summary for this kernel,
+    precision and size exists."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
-        m = re.match(r"^TRAFFIC precision=(\d+) n=(\d+) bytes_per_launch=([0-9.eE+]+)", line)
-        if m and int(m.group(1)) == precision and int(m.group(2)) == n:
-            return float(m.group(3))
+        m = re.match(r"^TRAFFIC kernel=(\w+) precision=(\d+) n=(\d+) bytes_per_launch=([0-9.eE+]+)", line)
+        if m and m.group(1) == kernel and int(m.group(2)) == precision and int(m.group(3)) == n:
+            return float(m.group(4))
     return None
 
 
@@ -184,6 +185,8 @@ def main():
         c = t_after[lab][1] - t_before[lab][1]
         phases[lab] = round(1e3 * s / max(1, a.steps), 4)
     adv_s = (t_after["advec4D"][0] - t_before["advec4D"][0]) / a.steps
+    variants = sim.kernel_variants()
+    adv_kernel = "k_advect4d_ys" if variants & 128 else "k_advect4d_blk"
     adv_bytes = advect4d_bytes(n, b)
     achieved = adv_bytes / adv_s / 1e9
     valu = ADVECT4D_FLOP_PER_CELL * n ** 4 / adv_s / 1e12
@@ -206,8 +209,9 @@ def main():
         "data": "synthetic: SLD10 Landau-damping initial condition generated on the GPU",
         "config": {"workload": f"vlp4d spline step, SLD10 deck at {n}^4 (box 12.56637112, v in [-9,9], dt 0.01)",
                    "grid": [n, n, n, n], "idcase": 10, "dt": 0.01, "storage_bits": 64 if a.precision == 64 else 32},
-        "roofline": {"bound": "hbm", "kernel": "k_advect4d_blk", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(a.precision, n),
+        "roofline": {"bound": "hbm", "kernel": adv_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(adv_kernel, a.precision, n),
                      "algorithmic_bytes_per_launch": adv_bytes, "avg_launch_ms": 1e3 * adv_s,
                      "valu": {"achieved": valu, "peak": VALU_PEAK_TFS, "unit": "TFLOP/s",
                               "frac": valu / VALU_PEAK_TFS, "flop_per_cell": ADVECT4D_FLOP_PER_CELL}},

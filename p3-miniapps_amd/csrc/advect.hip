@@ -326,26 +326,26 @@ static void launch_advect4d_a(const Geom &g, const T *c4, const double *ex, cons
   }
 }
 
-// The y-streaming kernel (advect_stream.hip) when the geometry fits it, then
-// the moments pass for rho and the L2 norm; otherwise the vx-blocked kernel
-// with both reductions fused.
+// The y-streaming kernel (advect_stream.hip) when the geometry fits it --
+// returns true: rho and the L2 norm are then still to be taken from fout by
+// launch_moments; otherwise the vx-blocked kernel with both reductions fused.
 template <>
-void launch_advect4d<double>(const Geom &g, const double *c4, const double *ex, const double *ey, double dt, bool,
+bool launch_advect4d<double>(const Geom &g, const double *c4, const double *ex, const double *ey, double dt, bool,
                              double *fout, double *rho_part, double *l2_part, int *err, hipStream_t s) {
-  if (launch_advect4d_ystream<double, double>(g, c4, ex, ey, dt, fout, err, s))
-    launch_moments<double>(g, fout, rho_part, l2_part, s);
-  else
-    launch_advect4d_a<double, double>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
+  if (launch_advect4d_ystream<double, double>(g, c4, ex, ey, dt, fout, err, s)) return true;
+  launch_advect4d_a<double, double>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
+  return false;
 }
 
 template <>
-void launch_advect4d<float>(const Geom &g, const float *c4, const double *ex, const double *ey, double dt,
+bool launch_advect4d<float>(const Geom &g, const float *c4, const double *ex, const double *ey, double dt,
                             bool arith32, float *fout, double *rho_part, double *l2_part, int *err, hipStream_t s) {
   const bool ys = arith32 ? launch_advect4d_ystream<float, float>(g, c4, ex, ey, dt, fout, err, s)
                           : launch_advect4d_ystream<float, double>(g, c4, ex, ey, dt, fout, err, s);
-  if (ys) launch_moments<float>(g, fout, rho_part, l2_part, s);
-  else if (arith32) launch_advect4d_a<float, float>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
+  if (ys) return true;
+  if (arith32) launch_advect4d_a<float, float>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
   else launch_advect4d_a<float, double>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
+  return false;
 }
 
 // moments of a given f: rho partials (sum over vx) and L2 partials

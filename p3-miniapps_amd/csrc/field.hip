@@ -40,6 +40,17 @@ __global__ __launch_bounds__(256) void k_rho_rfftx(const double *__restrict__ rh
   }
 }
 
+// Slice s of the density partials: tmp[s][i] = sum of planes o = s, s + S, ...
+__global__ __launch_bounds__(256) void k_rho_slices(const double *__restrict__ rho_part, long rho_outer, long npt,
+                                                    double *__restrict__ tmp) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y, S = gridDim.y;
+  if (i >= npt) return;
+  double acc = 0.;
+  for (long o = s; o < rho_outer; o += S) acc += rho_part[i + npt * o];
+  tmp[i + npt * s] = acc;
+}
+
 __global__ __launch_bounds__(256) void k_poisson_y(Geom g, FieldBufs b) {
   extern __shared__ double sh[];
   const int N0 = g.n[0], N1 = g.n[1];
@@ -230,6 +241,13 @@ void launch_field(const Geom &g, const FieldBufs &b, const double *rho_part, lon
                   long n_l2, int iter, bool record_diag, hipStream_t s) {
   const int N0 = g.n[0], N1 = g.n[1];
   const int nxh = N0 / 2 + 1;
+  if (rho_outer > 64 && b.rho_tmp) {
+    const long npt = (long)N0 * N1;
+    hipLaunchKernelGGL(k_rho_slices, dim3((unsigned)((npt + 255) / 256), RHO_SLICES), dim3(256), 0, s, rho_part,
+                       rho_outer, npt, b.rho_tmp);
+    rho_part = b.rho_tmp;
+    rho_outer = RHO_SLICES;
+  }
   if (g.idcase == 2 || g.idcase == 6) {
     hipLaunchKernelGGL(k_analytic_field, dim3(N1), dim3(256), 0, s, rho_part, rho_outer, g, b);
   } else {

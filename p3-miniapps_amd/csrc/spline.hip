@@ -24,6 +24,8 @@
 //                     matching instantiation.
 #include "spline_scan.hpp"
 
+#include <cstdlib>
+
 namespace vlp {
 
 #pragma clang fp contract(off)
@@ -257,6 +259,81 @@ __global__ __launch_bounds__(256) void k_spline_line_scan(const T *__restrict__ 
   }
 }
 
+// Strided lines through an LDS tile: the workgroup takes XT consecutive
+// contiguous-index positions i0 (a tile column each) of one outer index and
+// the whole line along the strided axis.  The N input rows of the tile (XT
+// values each, contiguous in memory) arrive by LDS-DMA, 16 bytes per lane;
+// the chunked-scan recursion runs down the tile columns (P lanes per column)
+// in place, leaving the N+4 coefficient rows, which go out as contiguous
+// segments.  Used when the map's i0 index is contiguous (m.is0 == m.os0 ==
+// 1); i1 and the outer index o select the tile's line family.  When the input
+// is contiguous across i1 as well (is1 == X0) the tiles run over the combined
+// index i0 + X0 i1 and each tile column finds its output position once.
+constexpr int TILE_XT = 64;
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+template <typename T, int C>
+__global__ __launch_bounds__(512) void k_spline_tile(const T *__restrict__ in, T *__restrict__ out, LineMap m, int P,
+                                                     SplineConsts sc, double aC, double aN3) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *tile = reinterpret_cast<T *>(smem_raw);  // [N+4][XT]
+  constexpr int XT = TILE_XT, VPL = 16 / (int)sizeof(T), CPR = XT / VPL;  // 16-byte chunks per row
+  const int N = P * C;
+  const bool comb = m.is1 == m.X0;
+  const long XC = comb ? m.X0 * m.X1 : m.X0;  // contiguous input extent
+  const long ntile = (XC + XT - 1) / XT;
+  const long tl = blockIdx.x % ntile, ob = blockIdx.x / ntile;
+  const long i1 = comb ? 0 : ob % m.X1, o = comb ? ob : ob / m.X1;
+  const long i0 = tl * XT;
+  const int nx = (int)((XC - i0) < XT ? (XC - i0) : XT);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  const T *src = in + i1 * m.is1 + o * m.ios + i0;
+  // rows k = 0..N-1 into tile rows 2..N+1: 64 lanes cover 64/CPR rows per DMA
+  {
+    const int rpi = 64 / CPR;  // rows per instruction
+    const int r = lane / CPR, c = lane % CPR;
+    for (int k0 = wave * rpi; k0 < N; k0 += nw * rpi) {
+      const int k = k0 + r;
+      if (k < N && c * VPL < nx)
+        __builtin_amdgcn_global_load_lds((const void *)(src + (long)k * m.ils + c * VPL),
+                                         (lds_void_t *)(tile + (long)(k0 + 2) * XT), 16, 0, 0);
+    }
+  }
+  __syncthreads();
+  // recursion down column col = tid / P, lane p = tid % P of the column group
+  {
+    const int col = tid / P, p = tid & (P - 1);
+    const bool act = col < nx;
+    T *cp = tile + (act ? col : 0);
+    double v[C];
+#pragma unroll
+    for (int i = 0; i < C; i++) v[i] = (double)cp[(2 + p * C + i) * XT];
+    double cm2, cm1, cN, cN1;
+    scan_recursion<C>(v, p, P, sc, aC, aN3, cm2, cm1, cN, cN1);
+    __syncthreads();  // all reads of the input rows done before rows are rewritten
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < C; i++) cp[(2 + p * C + i) * XT] = (T)v[i];
+      if (p == 0) {
+        cp[0] = (T)cm2;
+        cp[XT] = (T)cm1;
+      }
+      if (p == P - 1) {
+        cp[(N + 2) * XT] = (T)cN;
+        cp[(N + 3) * XT] = (T)cN1;
+      }
+    }
+  }
+  __syncthreads();
+  // N+4 coefficient rows out
+  T *dst = out + i1 * m.os1 + o * m.oos;
+  for (int x = lane; x < nx; x += 64) {
+    const long J = i0 + x;
+    const long ox = comb ? (J % m.X0) + (J / m.X0) * m.os1 : J;
+    for (int k = wave; k < N + 4; k += nw) dst[(long)k * m.ols + ox] = tile[k * XT + x];
+  }
+}
+
 // LDS fallback: one lane per line, the line in an LDS column of the lane.
 template <typename T>
 __global__ __launch_bounds__(64) void k_spline_line(const T *__restrict__ in, T *__restrict__ out, LineMap m,
@@ -303,8 +380,55 @@ static void launch_line_scan(const T *in, T *out, const LineMap &m, int P, int l
 #define VLP_LINE_CASE(CC) \
   case CC: return launch_line_scan<T, CC>(in, out, m, P, logP, s);
 
+template <typename T, int C>
+static void launch_tile(const T *in, T *out, const LineMap &m, int P, hipStream_t s) {
+  const SplineConsts sc = spline_consts();
+  double aC = 1.0, aN3 = 1.0;
+  for (int i = 0; i < C; i++) aC *= sc.alpha;
+  for (int i = 0; i < P * C + 3; i++) aN3 *= sc.alpha;
+  const bool comb = m.is1 == m.X0;
+  const long XC = comb ? m.X0 * m.X1 : m.X0;
+  const long ntile = (XC + TILE_XT - 1) / TILE_XT;
+  const size_t lds = (size_t)(P * C + 4) * TILE_XT * sizeof(T);
+  const void *fn = (const void *)k_spline_tile<T, C>;
+  if (lds > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_spline_tile<T, C>), dim3((unsigned)(ntile * (comb ? 1 : m.X1) * m.A)), dim3(TILE_XT * P), lds,
+                     s, in, out, m, P, sc, aC, aN3);
+}
+
+// Tile kernel when the contiguous index is i0 alone and the line splits into
+// P * C with C in [10, 16], P * 64 <= 512 lanes, the tile in LDS.
+static int tile_chunk(const LineMap &m, size_t tsz, int &P) {
+  if (const char *e = getenv("VLP4D_SPLINE_TILE"))
+    if (atoi(e) == 0) return 0;
+  if (m.is0 != 1 || m.os0 != 1) return 0;
+  int logP, C;
+  x_split(m.N, P, logP, C);
+  if (!C || P * TILE_XT > 512) return 0;
+  if ((size_t)(m.N + 4) * TILE_XT * tsz > 160 * 1024) return 0;
+  // input rows start 16-byte aligned (LDS-DMA of whole 16-byte chunks; the
+  // buffers themselves are 256-byte aligned)
+  if ((m.ils * tsz) % 16 || (m.is1 * tsz) % 16 || (m.ios * tsz) % 16) return 0;
+  const long XC = m.is1 == m.X0 ? m.X0 * m.X1 : m.X0;
+  if ((XC * (long)tsz) % 16) return 0;  // the last tile's chunks end inside the line family
+  return C;
+}
+
 template <typename T> void launch_spline_line(const T *in, T *out, const LineMap &m, hipStream_t s) {
   int P, logP, C;
+  {
+    int Pt;
+    switch (tile_chunk(m, sizeof(T), Pt)) {
+      case 10: return launch_tile<T, 10>(in, out, m, Pt, s);
+      case 11: return launch_tile<T, 11>(in, out, m, Pt, s);
+      case 12: return launch_tile<T, 12>(in, out, m, Pt, s);
+      case 13: return launch_tile<T, 13>(in, out, m, Pt, s);
+      case 14: return launch_tile<T, 14>(in, out, m, Pt, s);
+      case 15: return launch_tile<T, 15>(in, out, m, Pt, s);
+      case 16: return launch_tile<T, 16>(in, out, m, Pt, s);
+      default: break;
+    }
+  }
   line_split(m.N, 32, P, logP, C);
   switch (C) {
     VLP_LINE_CASE(10) VLP_LINE_CASE(11) VLP_LINE_CASE(12) VLP_LINE_CASE(13) VLP_LINE_CASE(14) VLP_LINE_CASE(15)

@@ -128,7 +128,7 @@ std::vector<double> twiddles(int n) {
 void free_all(vlp_ctx *c) {
   void *bufs[] = {c->f,        c->fnext,    c->cxy,       c->tv,     c->c4,      c->rho_part, c->l2_part,
                   c->fb.rho,   c->fb.ex,    c->fb.ey,     c->fb.spec1, c->fb.spec2, c->fb.twx,  c->fb.twy,
-                  c->fb.diag_part, c->fb.diag, c->d_err};
+                  c->fb.diag_part, c->fb.rho_tmp, c->fb.diag, c->d_err};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_err) (void)hipHostFree(c->h_err);
@@ -204,11 +204,21 @@ template <typename T> void run_advect2d(vlp_ctx *c) {
   c->rho_outer = c->g.n[3];
 }
 
-template <typename T> void run_advect4d(vlp_ctx *c) {
-  vlp::launch_advect4d<T>(c->g, (const T *)c->c4, c->fb.ex, c->fb.ey, c->cfg.dt, c->precision == VLP_FP32_ACC,
-                          (T *)c->fnext, c->rho_part, c->l2_part, c->d_err, c->stream);
+// Returns true when rho/L2 of fnext are still to be taken (run_moments_next).
+template <typename T> bool run_advect4d(vlp_ctx *c) {
+  const bool need = vlp::launch_advect4d<T>(c->g, (const T *)c->c4, c->fb.ex, c->fb.ey, c->cfg.dt,
+                                            c->precision == VLP_FP32_ACC, (T *)c->fnext, c->rho_part, c->l2_part,
+                                            c->d_err, c->stream);
   c->rho_outer = c->g.n[3];
   c->half_rho = false;
+  return need;
+}
+
+// rho and L2 partials of fnext (the reference's field_rho of fnp1 and
+// computeL2norm, VMO/Timestep.hpp:77-90).
+template <typename T> void run_moments_next(vlp_ctx *c) {
+  vlp::launch_moments<T>(c->g, (const T *)c->fnext, c->rho_part, c->l2_part, c->stream);
+  c->rho_outer = c->g.n[3];
 }
 
 template <typename T> void run_moments(vlp_ctx *c) {
@@ -263,8 +273,11 @@ int step_once(vlp_ctx *c) {
   HIP_TRY(c, hipEventRecord(ev[3], c->stream));
   if (f64) run_spline_vxvy<double>(c); else run_spline_vxvy<float>(c);
   HIP_TRY(c, hipEventRecord(ev[4], c->stream));
-  if (f64) run_advect4d<double>(c); else run_advect4d<float>(c);
+  const bool moments = f64 ? run_advect4d<double>(c) : run_advect4d<float>(c);
   HIP_TRY(c, hipEventRecord(ev[5], c->stream));
+  if (moments) {
+    if (f64) run_moments_next<double>(c); else run_moments_next<float>(c);
+  }
   rc = field(c, c->iter + 1, true, true);
   if (rc) return rc;
   HIP_TRY(c, hipEventRecord(ev[6], c->stream));
@@ -444,6 +457,7 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
   ALLOC(c->fb.twx, 2 * N0 * sizeof(double));
   ALLOC(c->fb.twy, 2 * N1 * sizeof(double));
   ALLOC(c->fb.diag_part, 4 * N1 * sizeof(double));
+  ALLOC(c->fb.rho_tmp, (size_t)vlp::RHO_SLICES * N0 * N1 * sizeof(double));
   ALLOC(c->fb.diag, 5 * (size_t)(cfg.nbiter + 1) * sizeof(double));
   ALLOC(c->d_err, sizeof(int));
 #undef ALLOC
@@ -637,7 +651,11 @@ int vlp_advect4d_poisson(vlp_ctx *c) {
   int rc = usable(c);
   if (rc) return rc;
   if (c->iter >= c->cfg.nbiter) return set_err(c, VLP_E_ARG, "iteration count would exceed nbiter");
-  if (c->precision == VLP_FP64) run_advect4d<double>(c); else run_advect4d<float>(c);
+  const bool f64 = c->precision == VLP_FP64;
+  const bool moments = f64 ? run_advect4d<double>(c) : run_advect4d<float>(c);
+  if (moments) {
+    if (f64) run_moments_next<double>(c); else run_moments_next<float>(c);
+  }
   rc = field(c, c->iter + 1, true, true);
   if (rc) return rc;
   std::swap(c->f, c->fnext);

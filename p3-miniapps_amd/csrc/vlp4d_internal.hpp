@@ -48,8 +48,10 @@ template <typename T> void launch_spline_line(const T *in, T *out, const LineMap
 template <typename T>
 void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, bool arith32, double *rho_part, int *err,
                         hipStream_t s);
+// Returns true when rho and the L2 norm of fout are still to be computed
+// (launch_moments); false when the kernel produced them.
 template <typename T>
-void launch_advect4d(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, bool arith32,
+bool launch_advect4d(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, bool arith32,
                      T *fout, double *rho_part, double *l2_part, int *err, hipStream_t s);
 // Fused spline x/y + 2-D half-step advection + density partials (xy_fused.hip):
 // f -> CXY and rho_part[grp][vy][y][x] for grp < G.  Returns false without
@@ -91,11 +93,14 @@ struct FieldBufs {
   double *spec2;   // 3 * (N0/2+1)*N1 complex: inverse along y (phi, ex, ey)
   double *twx, *twy;  // exp(-2 pi i m/N) tables, complex
   double *diag_part;  // 4*N1
+  double *rho_tmp;    // RHO_SLICES*N0*N1: slice sums of many density partials
   double *diag;       // 5*(nbiter+1): nrj, nrjx, nrjy, mass, l2
   int nbiter;
 };
 // rho_outer: number of N0*N1 density-partial planes in rho_part (N3, or
-// G*N3 after the fused xy kernel).
+// G*N3 after the fused xy kernel).  Beyond 64 planes they are first summed in
+// RHO_SLICES slices by a wide kernel.
+constexpr int RHO_SLICES = 16;
 void launch_field(const Geom &g, const FieldBufs &b, const double *rho_part, long rho_outer, const double *l2_part,
                   long n_l2, int iter, bool record_diag, hipStream_t s);
 
