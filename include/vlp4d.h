@@ -155,8 +155,10 @@ const char *vlp_timer_label(int which);
  * scan for the x spline, bit 1: multi-lane scan for the y/vx/vy splines,
  * bit 2: vx-blocked 4-D advection, bit 3: 4-cell blocks (else 2), bit 4: E
  * rows staged in LDS, bit 5: XCD-aware workgroup order, bit 6: advection
- * arithmetic in float).  No reference
- * counterpart: introspection for tests and reports. */
+ * arithmetic in float, bit 7: y-streaming 4-D advection (replaces the
+ * vx-blocked one), bits 8-11: its vx tile, bit 12: fused x/y spline + 2-D
+ * advection kernel (replaces the separate x, y and advect2d kernels)).  No
+ * reference counterpart: introspection for tests and reports. */
 int vlp_kernel_variants(const vlp_ctx *ctx);
 
 /* Message of the last error on this context (or the last global error when
