@@ -79,8 +79,13 @@ __device__ __forceinline__ void wait_vm(int n) {
 
 // Per-workgroup constants and per-lane state of the streaming loop.  The slab
 // is kept in the storage type T (moved by LDS-DMA, 16 bytes per lane) and
-// converted to the arithmetic type A as it is read.  E rows sit in LDS as
-// (Ex, Ey) pairs: [buffer][row k1 = 0..2][x].
+// converted to the arithmetic type A as it is read.  Its rows are contiguous
+// in LDS, so the slab is moved as one run of 16-byte chunks: DMA instruction j
+// fills LDS bytes [1024 j, 1024 j + 1024) of the buffer, lane l taking chunk
+// 64 j + l wherever it sits in C4; instructions are dealt to the waves in
+// turn, and each lane keeps the C4 byte offset (from the plane's first row)
+// of its chunk in every instruction of its wave.  E rows sit in LDS as
+// (Ex, Ey) pairs: [buffer][row k1 = 0..2][EPR], moved 64 points per instruction.
 //
 // computeFeet's 3x3 Lagrange interpolation of E at the half-step position
 // (x - dt/2 vx, y - dt/2 vy) separates: the x part (base column ip0 and
@@ -93,14 +98,20 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   // double arithmetic keeps each cell's fractional y position instead of its
   // four y weights (16 registers fewer per lane); float keeps the weights
   static constexpr bool WY = sizeof(A) == sizeof(double);
+  static constexpr int MS = 8;  // slab DMA instructions per wave per plane, at most
   const T *__restrict__ c4;
+  const double *__restrict__ exy;  // (Ex, Ey) pairs, [N1][N0]
   T *__restrict__ fout;
-  T *slab;      // [3][slab]
-  double *eb;   // [3][3][N0][2]
+  T *slab;         // NB buffers of slab_bytes
+  double *eb;      // [NB][3][EPR][2]
+  int slab_bytes;  // whole 1 KB blocks
+  int EPR, cpr;    // E points per LDS row (whole 64-point chunks), chunks per row
+  unsigned voff[MS];  // C4 byte offset of this lane's chunk in the wave's k-th slab DMA
+  int nslot;       // slab DMA instructions of this wave (wave-uniform)
   Geom g;
   FeetConsts fc;
   double dt;
-  int N0, N1, N2, N3, RX, slab_elems, nsteps, oyw;
+  int N0, N1, N2, N3, RX, nsteps, oyw;
   int ix, tv, ivx, ivy;
   bool active;
   bool wave_has_cells;  // some lane of this wave owns cells (wave-uniform)
@@ -110,49 +121,45 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   int wave, nwave, lane;
   int force;  // bit 0: every lane reads rows per cell; bit 1: every foot by direct gather
   int cbase;  // C4 element of (x -2, y -3, vx tile-2, vy ivy-2)
-  const unsigned *esrc;  // E DMA source of this lane: Ex or Ey dword of its point within a 16-point chunk
   double cx[3];          // x Lagrange weights of the half-step foot
   int jx[3];             // wrapped E columns ip0 .. ip0+2
   double gx[3], gy[3];   // x-interpolated E of rows ip1 .. ip1+2
-  int ip1p;              // ip1 of the previous step
+  int ip1p;              // ip1 of the previous step (wave-uniform)
   OpenCell<A> cell[4];
   int bad;
 
-  // E row ip1 of the cell y (computeFeet: round of the half-step y) - 1.
+  // E row ip1 of the cell y (computeFeet: round of the half-step y) - 1;
+  // uniform in the workgroup.
   __device__ __forceinline__ int erow0(int y) const {
     const double yy = g.minPhy[1] + y * g.dx[1];
-    return (int)(round(half_y(g, yy, vy, dt)) - 1);
+    return __builtin_amdgcn_readfirstlane((int)(round(half_y(g, yy, vy, dt)) - 1));
   }
 
-  // Start moving step q's slab and E rows into LDS buffer b by LDS-DMA:
-  // slab rows (row = kvx + (TVX+4) kvy: vx tile-2+kvx, vy ivy-2+kvy) and
-  // 16-point chunks of the three E rows are dealt to the waves in turn.
+  // Start moving step q's slab and E rows into LDS buffer b by LDS-DMA.
   // Returns the number of DMA instructions this wave issued (wave-uniform).
   __device__ __forceinline__ int fetch(int q, int b) {
     const int p = q + oyw - 1;  // y-position of the plane
-    const T *src = c4 + cbase + F0 * (p + 3);
-    T *sb = slab + b * slab_elems;
-    const int per_row = RX / VPL;
-    const long F01 = (long)F0 * (N1 + 6);
+    const char *src = (const char *)c4 + ((long)cbase + (long)F0 * (p + 3)) * (long)sizeof(T);
+    char *sb = (char *)slab + b * slab_bytes;
     int n = 0;
-    for (int row = wave; row < 5 * (TVX + 4); row += nwave) {
-      const int kvx = row % (TVX + 4), kvy = row / (TVX + 4);
-      const T *rs = src + F01 * (kvx + (long)(N2 + 6) * kvy);
-      T *rl = sb + row * RX;
-      for (int c0 = 0; c0 < per_row; c0 += 64, n++)
-        if (c0 + lane < per_row)
-          __builtin_amdgcn_global_load_lds((const void *)(rs + VPL * (c0 + lane)), (lds_void *)(rl + VPL * c0), 16, 0,
-                                           0);
-    }
+#pragma unroll
+    for (int k = 0; k < MS; k++)
+      if (k < nslot) {
+        __builtin_amdgcn_global_load_lds((const void *)(src + voff[k]), (lds_void *)(sb + 1024 * (wave + k * nwave)),
+                                         16, 0, 0);
+        n++;
+      }
     if (q < N1) {
       const int ip1 = erow0(q);
-      const int cpr = N0 / 16;  // chunks per E row
-      unsigned *el = (unsigned *)(eb + b * 6 * N0);
-      for (int ch = wave; ch < 3 * cpr; ch += nwave, n++) {
-        const int k1 = ch / cpr, c = ch - k1 * cpr;
-        const int row = ((ip1 + k1) % N1 + N1) % N1;
-        __builtin_amdgcn_global_load_lds((const void *)(esrc + 2 * (row * N0 + 16 * c)), (lds_void *)(el + 64 * ch), 4,
-                                         0, 0);
+      char *el = (char *)(eb + b * 6 * EPR);
+      for (int e = wave; e < 3 * cpr; e += nwave, n++) {
+        const int k1 = (e >= cpr) + (e >= 2 * cpr), c = e - k1 * cpr;
+        int row = ip1 + k1;
+        if (row < 0 || row >= N1) row = (row % N1 + N1) % N1;
+        // points beyond N0 (last chunk) repeat the row's last point into the padding
+        const int pt = min(64 * c + lane, N0 - 1);
+        __builtin_amdgcn_global_load_lds((const void *)(exy + 2 * ((long)row * N0 + pt)),
+                                         (lds_void *)(el + 16 * (k1 * EPR + 64 * c)), 16, 0, 0);
       }
     }
     return n;
@@ -163,24 +170,24 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   __device__ __forceinline__ void field_rows(int q, int b, double cy[3], double &yq) {
     yq = g.minPhy[1] + q * g.dx[1];
     const double xt1 = half_y(g, yq, vy, dt);
-    const int ip1 = (int)(round(xt1) - 1);
+    const int ip1 = __builtin_amdgcn_readfirstlane((int)(round(xt1) - 1));
     const double py = xt1 - ip1;
     cy[0] = .5 * (py - 1.) * (py - 2.);
     cy[1] = -1. * (py) * (py - 2.);
     cy[2] = .5 * (py) * (py - 1.);
-    const double2 *er = (const double2 *)(eb + b * 6 * N0);
+    const double2 *er = (const double2 *)(eb + b * 6 * EPR);
     if (ip1 == ip1p + 1) {
       gx[0] = gx[1];
       gy[0] = gy[1];
       gx[1] = gx[2];
       gy[1] = gy[2];
-      const double2 e0 = er[2 * N0 + jx[0]], e1 = er[2 * N0 + jx[1]], e2 = er[2 * N0 + jx[2]];
+      const double2 e0 = er[2 * EPR + jx[0]], e1 = er[2 * EPR + jx[1]], e2 = er[2 * EPR + jx[2]];
       gx[2] = cx[0] * e0.x + cx[1] * e1.x + cx[2] * e2.x;
       gy[2] = cx[0] * e0.y + cx[1] * e1.y + cx[2] * e2.y;
     } else {
 #pragma unroll
       for (int k1 = 0; k1 < 3; k1++) {
-        const double2 e0 = er[k1 * N0 + jx[0]], e1 = er[k1 * N0 + jx[1]], e2 = er[k1 * N0 + jx[2]];
+        const double2 e0 = er[k1 * EPR + jx[0]], e1 = er[k1 * EPR + jx[1]], e2 = er[k1 * EPR + jx[2]];
         gx[k1] = cx[0] * e0.x + cx[1] * e1.x + cx[2] * e2.x;
         gy[k1] = cx[0] * e0.y + cx[1] * e1.y + cx[2] * e2.y;
       }
@@ -272,7 +279,7 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   // Add plane q (LDS buffer b) to the four open cells; the cell opened at
   // step q - j sits in slot (R - j) & 3 and takes y-weight j.
   template <int R> __device__ __forceinline__ void accumulate(int b) {
-    const T *s = slab + b * slab_elems;
+    const T *s = (const T *)((const char *)slab + b * slab_bytes);
     const int VS = (TVX + 4) * RX;  // vy row stride
     int k0 = -1;
     bool shared = !(force & 1);
@@ -397,13 +404,14 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
 }  // namespace
 
 template <typename T, typename A, int TVX, int NB>
-__global__ __launch_bounds__(512) void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ ex,
-                                                     const double *__restrict__ ey, Geom g, FeetConsts fc, double dt,
-                                                     T *__restrict__ fout, int *__restrict__ err, int force) {
+__global__ __launch_bounds__(512) void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geom g,
+                                                     FeetConsts fc, double dt, T *__restrict__ fout,
+                                                     int *__restrict__ err, int force) {
   extern __shared__ __align__(16) unsigned char ys_smem[];
   YStream<T, A, TVX, NB> s;
   s.force = force;
   s.c4 = c4;
+  s.exy = exy;
   s.fout = fout;
   s.g = g;
   s.fc = fc;
@@ -413,9 +421,14 @@ __global__ __launch_bounds__(512) void k_advect4d_ys(const T *__restrict__ c4, c
   s.N2 = g.n[2];
   s.N3 = g.n[3];
   s.RX = s.N0 + 4;
-  s.slab_elems = 5 * (TVX + 4) * s.RX;
+  const int per_row = s.RX / s.VPL;                 // 16-byte chunks per slab row
+  const int nchunk = 5 * (TVX + 4) * per_row;
+  const int ninstr = (nchunk + 63) >> 6;
+  s.slab_bytes = ninstr * 1024;
+  s.cpr = (s.N0 + 63) >> 6;
+  s.EPR = 64 * s.cpr;
   s.slab = (T *)ys_smem;
-  s.eb = (double *)(ys_smem + NB * (size_t)s.slab_elems * sizeof(T));
+  s.eb = (double *)(ys_smem + NB * (size_t)s.slab_bytes);
   s.nsteps = s.N1 + 3;
   const int ntile = s.N2 / TVX;
   const int tile = blockIdx.x % ntile;
@@ -436,8 +449,18 @@ __global__ __launch_bounds__(512) void k_advect4d_ys(const T *__restrict__ c4, c
   s.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   s.lane = tid & 63;
   s.nwave = (int)(blockDim.x >> 6);
-  // E DMA: lane l moves dword l % 2 of component (l % 4) / 2 of point l / 4
-  s.esrc = (const unsigned *)(((s.lane & 3) >> 1) ? ey : ex) + 2 * (s.lane >> 2) + (s.lane & 1);
+  s.nslot = s.wave < ninstr ? (ninstr - 1 - s.wave) / s.nwave + 1 : 0;
+  {
+    const long F01 = (long)s.F0 * (s.N1 + 6);
+#pragma unroll
+    for (int k = 0; k < s.MS; k++) {
+      // chunks past the slab repeat its last chunk into the buffer's padding
+      const int c = min(64 * (s.wave + k * s.nwave) + s.lane, nchunk - 1);
+      const int row = c / per_row, col = c - row * per_row;
+      const int kvx = row % (TVX + 4), kvy = row / (TVX + 4);
+      s.voff[k] = (unsigned)((F01 * (kvx + (long)(s.N2 + 6) * kvy) + (long)s.VPL * col) * (long)sizeof(T));
+    }
+  }
   {  // x part of computeFeet's E interpolation (VMO/Advection.hpp:174-196)
     double xt0 = s.x - 0.5 * dt * s.vx;
     xt0 = g.inv_dx[0] * (xt0 - g.minPhy[0]);
@@ -495,6 +518,11 @@ struct YsPlan {
   size_t lds = 0;
 };
 
+// Slab DMA instructions per plane and E points per LDS row of the streaming
+// kernel (the same arithmetic as its set-up).
+long ys_slab_instr(long N0, long tvx, size_t tsz) { return (5 * (tvx + 4) * ((N0 + 4) * (long)tsz / 16) + 63) / 64; }
+long ys_epr(long N0) { return (N0 + 63) / 64 * 64; }
+
 YsPlan ys_plan(const Geom &g, size_t tsz) {
   YsPlan p;
   const long N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
@@ -517,9 +545,10 @@ YsPlan ys_plan(const Geom &g, size_t tsz) {
     const long lanes = N0 * tvx;
     if (lanes > 512) continue;
     const int block = (int)((lanes + 63) / 64 * 64);
-    const long slab = 5L * (tvx + 4) * (N0 + 4);
-    if ((N0 + 4) % (16 / (long)tsz) || N0 % 16) continue;  // whole 16-byte slab chunks, 16-point E chunks
-    const size_t lds = nb * (size_t)slab * tsz + nb * 6 * (size_t)N0 * sizeof(double);
+    if ((N0 + 4) % (16 / (long)tsz) || N0 % 16) continue;  // whole 16-byte slab chunks per row
+    const long ninstr = ys_slab_instr(N0, tvx, tsz);
+    if ((ninstr + block / 64 - 1) / (block / 64) > 8) continue;  // DMA slots per wave
+    const size_t lds = nb * (size_t)ninstr * 1024 + nb * 3 * (size_t)ys_epr(N0) * 2 * sizeof(double);
     if (lds > 160 * 1024) continue;
     p.tvx = tvx;
     p.nb = nb;
@@ -531,22 +560,22 @@ YsPlan ys_plan(const Geom &g, size_t tsz) {
 }
 
 template <typename T, typename A, int TVX, int NB>
-void launch_ys_t(const Geom &g, const YsPlan &p, const T *c4, const double *ex, const double *ey, double dt, T *fout,
-                 int *err, hipStream_t s) {
+void launch_ys_t(const Geom &g, const YsPlan &p, const T *c4, const double *exy, double dt, T *fout, int *err,
+                 hipStream_t s) {
   const void *fn = (const void *)k_advect4d_ys<T, A, TVX, NB>;
   if (p.lds > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
   const unsigned grid = (unsigned)((g.n[2] / TVX) * g.n[3]);
   int force = 0;  // test switch: VLP4D_YS_FORCE=1 rows per cell, 2 direct gather, 3 both
   if (const char *e = getenv("VLP4D_YS_FORCE")) force = atoi(e) & 3;
-  hipLaunchKernelGGL((k_advect4d_ys<T, A, TVX, NB>), dim3(grid), dim3(p.block), p.lds, s, c4, ex, ey, g,
+  hipLaunchKernelGGL((k_advect4d_ys<T, A, TVX, NB>), dim3(grid), dim3(p.block), p.lds, s, c4, exy, g,
                      feet_consts(g), dt, fout, err, force);
 }
 
 template <typename T, typename A, int TVX>
-void launch_ys_nb(const Geom &g, const YsPlan &p, const T *c4, const double *ex, const double *ey, double dt,
-                  T *fout, int *err, hipStream_t s) {
-  if (p.nb == 2) launch_ys_t<T, A, TVX, 2>(g, p, c4, ex, ey, dt, fout, err, s);
-  else launch_ys_t<T, A, TVX, 3>(g, p, c4, ex, ey, dt, fout, err, s);
+void launch_ys_nb(const Geom &g, const YsPlan &p, const T *c4, const double *exy, double dt, T *fout, int *err,
+                  hipStream_t s) {
+  if (p.nb == 2) launch_ys_t<T, A, TVX, 2>(g, p, c4, exy, dt, fout, err, s);
+  else launch_ys_t<T, A, TVX, 3>(g, p, c4, exy, dt, fout, err, s);
 }
 
 }  // namespace
@@ -554,15 +583,15 @@ void launch_ys_nb(const Geom &g, const YsPlan &p, const T *c4, const double *ex,
 // Returns false (nothing launched) when the geometry does not fit the
 // streaming kernel.
 template <typename T, typename A>
-bool launch_advect4d_ystream(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, T *fout,
-                             int *err, hipStream_t s) {
+bool launch_advect4d_ystream(const Geom &g, const T *c4, const double *exy, double dt, T *fout, int *err,
+                             hipStream_t s) {
   if (const char *e = getenv("VLP4D_YSTREAM"))
     if (atoi(e) == 0) return false;
   const YsPlan p = ys_plan(g, sizeof(T));
   switch (p.tvx) {
-    case 8: launch_ys_nb<T, A, 8>(g, p, c4, ex, ey, dt, fout, err, s); return true;
-    case 4: launch_ys_nb<T, A, 4>(g, p, c4, ex, ey, dt, fout, err, s); return true;
-    case 2: launch_ys_nb<T, A, 2>(g, p, c4, ex, ey, dt, fout, err, s); return true;
+    case 8: launch_ys_nb<T, A, 8>(g, p, c4, exy, dt, fout, err, s); return true;
+    case 4: launch_ys_nb<T, A, 4>(g, p, c4, exy, dt, fout, err, s); return true;
+    case 2: launch_ys_nb<T, A, 2>(g, p, c4, exy, dt, fout, err, s); return true;
     default: return false;
   }
 }
@@ -574,11 +603,11 @@ int ystream_tvx(const Geom &g, bool fp32_storage, bool arith32) {
   return ys_plan(g, fp32_storage ? sizeof(float) : sizeof(double)).tvx;
 }
 
-template bool launch_advect4d_ystream<double, double>(const Geom &, const double *, const double *, const double *,
-                                                      double, double *, int *, hipStream_t);
-template bool launch_advect4d_ystream<float, double>(const Geom &, const float *, const double *, const double *,
-                                                     double, float *, int *, hipStream_t);
-template bool launch_advect4d_ystream<float, float>(const Geom &, const float *, const double *, const double *,
-                                                    double, float *, int *, hipStream_t);
+template bool launch_advect4d_ystream<double, double>(const Geom &, const double *, const double *, double, double *,
+                                                      int *, hipStream_t);
+template bool launch_advect4d_ystream<float, double>(const Geom &, const float *, const double *, double, float *,
+                                                     int *, hipStream_t);
+template bool launch_advect4d_ystream<float, float>(const Geom &, const float *, const double *, double, float *, int *,
+                                                    hipStream_t);
 
 }  // namespace vlp

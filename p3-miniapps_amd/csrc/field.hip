@@ -159,6 +159,8 @@ __global__ __launch_bounds__(256) void k_c2rx_diag(Geom g, FieldBufs b) {
     b.rho[id] = out[0];
     b.ex[id] = out[1];
     b.ey[id] = out[2];
+    b.exy[2 * id] = out[1];
+    b.exy[2 * id + 1] = out[2];
     pm += out[0];
     pe += out[1] * out[1] + out[2] * out[2];
     pex += out[1] * out[1];
@@ -192,6 +194,8 @@ __global__ __launch_bounds__(256) void k_analytic_field(const double *__restrict
     b.rho[id] = r;
     b.ex[id] = e;
     b.ey[id] = 0.;
+    b.exy[2 * id] = e;
+    b.exy[2 * id + 1] = 0.;
     pm += r;
     pe += e * e;
     pex += e * e;

@@ -127,7 +127,7 @@ std::vector<double> twiddles(int n) {
 
 void free_all(vlp_ctx *c) {
   void *bufs[] = {c->f,        c->fnext,    c->cxy,       c->tv,     c->c4,      c->rho_part, c->l2_part,
-                  c->fb.rho,   c->fb.ex,    c->fb.ey,     c->fb.spec1, c->fb.spec2, c->fb.twx,  c->fb.twy,
+                  c->fb.rho,   c->fb.ex,    c->fb.ey,     c->fb.exy, c->fb.spec1, c->fb.spec2, c->fb.twx,  c->fb.twy,
                   c->fb.diag_part, c->fb.rho_tmp, c->fb.diag, c->d_err};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
@@ -206,7 +206,7 @@ template <typename T> void run_advect2d(vlp_ctx *c) {
 
 // Returns true when rho/L2 of fnext are still to be taken (run_moments_next).
 template <typename T> bool run_advect4d(vlp_ctx *c) {
-  const bool need = vlp::launch_advect4d<T>(c->g, (const T *)c->c4, c->fb.ex, c->fb.ey, c->cfg.dt,
+  const bool need = vlp::launch_advect4d<T>(c->g, (const T *)c->c4, c->fb.ex, c->fb.ey, c->fb.exy, c->cfg.dt,
                                             c->precision == VLP_FP32_ACC, (T *)c->fnext, c->rho_part, c->l2_part,
                                             c->d_err, c->stream);
   c->rho_outer = c->g.n[3];
@@ -452,6 +452,7 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
   ALLOC(c->fb.rho, N0 * N1 * sizeof(double));
   ALLOC(c->fb.ex, N0 * N1 * sizeof(double));
   ALLOC(c->fb.ey, N0 * N1 * sizeof(double));
+  ALLOC(c->fb.exy, 2 * N0 * N1 * sizeof(double));
   ALLOC(c->fb.spec1, 2 * nxh * N1 * sizeof(double));
   ALLOC(c->fb.spec2, 3 * 2 * nxh * N1 * sizeof(double));
   ALLOC(c->fb.twx, 2 * N0 * sizeof(double));
@@ -472,7 +473,8 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
       hipMemset(c->c4, 0, c->n_c4s * c->esz) != hipSuccess ||
       hipMemset(c->fb.rho, 0, N0 * N1 * sizeof(double)) != hipSuccess ||
       hipMemset(c->fb.ex, 0, N0 * N1 * sizeof(double)) != hipSuccess ||
-      hipMemset(c->fb.ey, 0, N0 * N1 * sizeof(double)) != hipSuccess)
+      hipMemset(c->fb.ey, 0, N0 * N1 * sizeof(double)) != hipSuccess ||
+      hipMemset(c->fb.exy, 0, 2 * N0 * N1 * sizeof(double)) != hipSuccess)
     return fail(set_err(nullptr, VLP_E_HIP, "device initialisation failed"));
   // init(): VMO/Init.cpp:370-372 then VMO/Vlp4d.cpp:30-34
   const auto t0 = std::chrono::steady_clock::now();

@@ -51,8 +51,8 @@ void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, bool arith
 // Returns true when rho and the L2 norm of fout are still to be computed
 // (launch_moments); false when the kernel produced them.
 template <typename T>
-bool launch_advect4d(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, bool arith32,
-                     T *fout, double *rho_part, double *l2_part, int *err, hipStream_t s);
+bool launch_advect4d(const Geom &g, const T *c4, const double *ex, const double *ey, const double *exy, double dt,
+                     bool arith32, T *fout, double *rho_part, double *l2_part, int *err, hipStream_t s);
 // Fused spline x/y + 2-D half-step advection + density partials (xy_fused.hip):
 // f -> CXY and rho_part[grp][vy][y][x] for grp < G.  Returns false without
 // launching when the geometry does not fit; xy_fused_groups gives G (0: the
@@ -64,8 +64,8 @@ int xy_fused_groups(const Geom &g, size_t tsz);
 // y-streaming 4-D advection (f only; rho and L2 by launch_moments afterwards).
 // Returns false without launching when the geometry does not fit it.
 template <typename T, typename A>
-bool launch_advect4d_ystream(const Geom &g, const T *c4, const double *ex, const double *ey, double dt, T *fout,
-                             int *err, hipStream_t s);
+bool launch_advect4d_ystream(const Geom &g, const T *c4, const double *exy, double dt, T *fout, int *err,
+                             hipStream_t s);
 // vx tile of the y-streaming kernel for a geometry (0: not used).
 int ystream_tvx(const Geom &g, bool fp32_storage, bool arith32);
 template <typename T>
@@ -89,6 +89,7 @@ int kernel_variants(const Geom &g, bool fp32_storage, bool arith32);
 struct FieldBufs {
   double *rho;     // N0*N1 : rho, then phi (Poisson cases)
   double *ex, *ey; // N0*N1
+  double *exy;     // N0*N1 (Ex, Ey) pairs, the layout the streaming advection stages
   double *spec1;   // (N0/2+1)*N1 complex: r2c along x
   double *spec2;   // 3 * (N0/2+1)*N1 complex: inverse along y (phi, ex, ey)
   double *twx, *twy;  // exp(-2 pi i m/N) tables, complex
