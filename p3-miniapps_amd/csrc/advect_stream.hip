@@ -17,11 +17,11 @@
 // step and per lane, 340 FMAs per cell, no re-reads from L2.
 //
 // A lane reads one set of rows for its four cells when they agree on their x,
-// vx and vy stencil offsets (each in {-1, 0} relative to the cell), else each
-// cell reads its own rows.  A foot outside this pattern (displacement beyond
-// one cell, or y offset different from the workgroup's) is interpolated by the
-// direct 4^4 gather from C4 in global memory at the step it is opened; its
-// streamed weights are zero.
+// vx and vy stencil offsets (each in {-1, 0} relative to the cell), else one
+// set per distinct offset pattern.  A foot outside this pattern (displacement
+// beyond one cell, or y offset different from the workgroup's) is interpolated
+// by the direct 4^4 gather from C4 in global memory at the step it is opened
+// and takes no streamed contribution.
 #include "advect_common.hpp"
 
 #include <cstdlib>
@@ -196,18 +196,14 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   }
 
   // Open cell y = q in slot R: the rest of computeFeet (VMO/Advection.hpp:
-  // 166-218), the bound check, B-spline weights and stencil offsets.
+  // 166-218), the bound check, B-spline weights and stencil offsets.  A cell
+  // that takes no streamed contribution gets key -1; accumulate() then never
+  // adds to it, whatever its weights hold.
   template <int R> __device__ __forceinline__ void open(int q, double yq, const double cy[3]) {
     OpenCell<A> &c = cell[R];
     c.acc = 0;
     c.key = -1;
-    if (q >= N1) {
-#pragma unroll
-      for (int j = 0; j < 4; j++)
-#pragma unroll
-        for (int k = 0; k < 4; k++) c.w[j][k] = 0;
-      return;
-    }
+    if (q >= N1) return;
     const double ftmp1 = cy[0] * gx[0] + cy[1] * gx[1] + cy[2] * gx[2];
     const double ftmp2 = cy[0] * gy[0] + cy[1] * gy[1] + cy[2] * gy[2];
     const double xt2 = vx - 0.5 * dt * ftmp1;
@@ -225,9 +221,10 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
       const double m = (hi < a) ? hi : a;
       xs[3] = (m < lo) ? lo : m;
     }
+    // all eight comparisons, no short cut (NaN fails them)
     bool ok = true;
 #pragma unroll
-    for (int j = 0; j < 4; j++) ok = ok && (xs[j] >= fc.locmin[j] && xs[j] <= fc.locmax[j]);
+    for (int j = 0; j < 4; j++) ok = ok & (xs[j] >= fc.locmin[j]) & (xs[j] <= fc.locmax[j]);
     int ip[4];
     A wyk[4];  // y weights, kept only for a direct gather
 #pragma unroll
@@ -245,28 +242,21 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
       }
     }
     const int ox = ip[0] - ix, oy = ip[1] - q, ovx = ip[2] - ivx, ovy = ip[3] - ivy;
-    const bool fast = ok && (ox == -1 || ox == 0) && oy == oyw && (ovx == -1 || ovx == 0) &&
-                      (ovy == -1 || ovy == 0) && !(force & 2);
-    c.key = (ox + 1) | ((ovx + 1) << 1) | ((ovy + 1) << 2);
-    if (!fast) {
-      // out of domain (the reference stops with "Time step is too large"), or
-      // a foot outside the streamed pattern: direct gather, nothing streamed
-      if (ok) {
-        // a copy keeps the slot's weights out of the callee's reach (the
-        // per-lane state then stays in registers)
-        A e[4][4];
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-          for (int k = 0; k < 4; k++) e[j][k] = j == 1 ? wyk[k] : c.w[j][k];
-        c.acc = gather4d<T, A>(c4, S1, S2, S3, ip, e);
-      }
-      else bad = 1;
-      c.key = -1;
+    const bool fast = ok & ((unsigned)(ox + 1) < 2u) & (oy == oyw) & ((unsigned)(ovx + 1) < 2u) &
+                      ((unsigned)(ovy + 1) < 2u) & !(force & 2);
+    c.key = fast ? (ox + 1) | ((ovx + 1) << 1) | ((ovy + 1) << 2) : -1;
+    // out of domain: the reference stops with "Time step is too large"
+    bad |= !ok;
+    if (ok & !fast) {
+      // a foot outside the streamed pattern: direct gather.  A copy keeps the
+      // slot's weights out of the callee's reach (the per-lane state then
+      // stays in registers).
+      A e[4][4];
 #pragma unroll
       for (int j = 0; j < 4; j++)
 #pragma unroll
-        for (int k = 0; k < 4; k++) c.w[j][k] = 0;
+        for (int k = 0; k < 4; k++) e[j][k] = j == 1 ? wyk[k] : c.w[j][k];
+      c.acc = gather4d<T, A>(c4, S1, S2, S3, ip, e);
     }
   }
 
@@ -277,27 +267,33 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   }
 
   // Add plane q (LDS buffer b) to the four open cells; the cell opened at
-  // step q - j sits in slot (R - j) & 3 and takes y-weight j.
+  // step q - j sits in slot (R - j) & 3 and takes y-weight j.  One pass reads
+  // the 16 rows of one stencil key per lane (the first key among its cells
+  // still to do) and adds to the cells with that key; the wave repeats until
+  // no lane has cells left (one pass when a lane's cells agree, the usual
+  // case).  force & 1 takes one cell per pass.
   template <int R> __device__ __forceinline__ void accumulate(int b) {
     const T *s = (const T *)((const char *)slab + b * slab_bytes);
     const int VS = (TVX + 4) * RX;  // vy row stride
-    int k0 = -1;
-    bool shared = !(force & 1);
+    unsigned todo = 0;
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const int k = cell[c].key;
-      if (k >= 0) {
-        if (k0 < 0) k0 = k;
-        else shared = shared && (k == k0);
-      }
-    }
-    A t[4], u[4];
+    for (int c = 0; c < 4; c++) todo |= (cell[c].key >= 0 ? 1u : 0u) << c;
+    while (__builtin_amdgcn_ballot_w64(todo != 0)) {
+      int k0 = 0;
 #pragma unroll
-    for (int c = 0; c < 4; c++) t[c] = u[c] = 0;
-    if (shared) {
+      for (int c = 3; c >= 0; c--)
+        if ((todo >> c) & 1) k0 = cell[c].key;
+      unsigned sel = 0;
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (((todo >> c) & 1) && cell[c].key == k0 && !((force & 1) && sel)) sel |= 1u << c;
+      todo &= ~sel;
+      A t[4], u[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) t[c] = u[c] = 0;
       // rows (kvx, kvy) in order, the next row's LDS reads issued before the
       // current row's FMAs (one row of look-ahead hides the LDS latency)
-      const T *sp = s + row_base(k0 < 0 ? 0 : k0);
+      const T *sp = s + row_base(k0);
       T r[2][4];
 #pragma unroll
       for (int k = 0; k < 4; k++) r[0][k] = sp[k];
@@ -329,40 +325,20 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-    } else {
-      int bs[4];
-#pragma unroll
-      for (int c = 0; c < 4; c++) bs[c] = row_base(cell[c].key < 0 ? 0 : cell[c].key);
-#pragma unroll
-      for (int kvy = 0; kvy < 4; kvy++) {
-#pragma unroll
-        for (int kvx = 0; kvx < 4; kvx++) {
-#pragma unroll
-          for (int c = 0; c < 4; c++) {
-            const T *rp = s + bs[c] + kvy * VS + kvx * RX;
-            const A *w0 = cell[c].w[0];
-            const A d = w0[0] * (A)rp[0] + w0[1] * (A)rp[1] + w0[2] * (A)rp[2] + w0[3] * (A)rp[3];
-            u[c] += cell[c].w[2][kvx] * d;
-          }
-        }
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-          t[c] += cell[c].w[3][kvy] * u[c];
-          u[c] = 0;
-        }
-      }
+      close_plane<R, 0>(t, sel);
+      close_plane<R, 1>(t, sel);
+      close_plane<R, 2>(t, sel);
+      close_plane<R, 3>(t, sel);
     }
-    close_plane<R, 0>(t);
-    close_plane<R, 1>(t);
-    close_plane<R, 2>(t);
-    close_plane<R, 3>(t);
   }
 
   // The cell opened J steps ago takes the plane's contribution with its y
-  // weight J.
-  template <int R, int J> __device__ __forceinline__ void close_plane(const A t[4]) {
-    OpenCell<A> &c = cell[(R - J) & 3];
-    c.acc += (WY ? bspline_eta_j<A, J>(c.wy) : c.w[1][J]) * t[(R - J) & 3];
+  // weight J, when this pass selected it.
+  template <int R, int J> __device__ __forceinline__ void close_plane(const A t[4], unsigned sel) {
+    constexpr int c = (R - J) & 3;
+    OpenCell<A> &oc = cell[c];
+    const A add = (WY ? bspline_eta_j<A, J>(oc.wy) : oc.w[1][J]) * t[c];
+    oc.acc += ((sel >> c) & 1) ? add : (A)0;
   }
 
   // Write cell y = q - 3 (slot (R + 1) & 3).
@@ -404,7 +380,8 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
 }  // namespace
 
 template <typename T, typename A, int TVX, int NB>
-__global__ __launch_bounds__(512) void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geom g,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(A) == 4 ? 3 : 1)))
+void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geom g,
                                                      FeetConsts fc, double dt, T *__restrict__ fout,
                                                      int *__restrict__ err, int force) {
   extern __shared__ __align__(16) unsigned char ys_smem[];
