@@ -153,21 +153,25 @@ def test_advect4d_variants_match_oracle(vlp, orc, monkeypatch, n, L, elds, remap
     assert relmax(d["nrj"], st.diags()["nrj"]) < 1e-12
 
 
-# The y-streaming 4-D advection: lanes whose four open cells agree on their
-# stencil offsets share one set of LDS rows, others read rows per cell, and
-# feet outside the streamed pattern (here: the vy = 0 slab, whose y offset
-# follows the sign of E) take the direct gather.  VLP4D_YS_FORCE makes every
-# lane take the per-cell rows (1), every foot the direct gather (2), or both
-# (3); each path must give the oracle's result, with two LDS plane buffers
-# (the next plane's loads overlap the step) and with three (loads two steps
-# ahead).  Shapes: a vy = 0 slab (Nvy even), vx tiles of 8 and 4, Nx a
-# multiple of 16 but not of 64.
+# The y-streaming 4-D advection: a lane adds the plane to its four open cells
+# in one pass over the LDS rows when their stencil offsets agree, in one pass
+# per distinct offset pattern otherwise; feet outside the streamed pattern
+# (here: the vy = 0 slab, whose y offset follows the sign of E) take the
+# direct gather.  VLP4D_YS_FORCE makes every lane take one pass per cell (1),
+# every foot the direct gather (2), or both (3); each path must give the
+# oracle's result, with two LDS plane buffers (the next plane's loads overlap
+# the step) and with three (loads two steps ahead), at the default vx tile
+# and at tile 4.  Shapes: a vy = 0 slab (Nvy even), Nx a multiple of 16 but
+# not of 64, several waves per workgroup and one.
 @pytest.mark.parametrize("n", [(32, 12, 20, 14), (48, 20, 16, 18), (64, 16, 32, 12)])
 @pytest.mark.parametrize("force", [0, 1, 2, 3])
 @pytest.mark.parametrize("nbuf", [2, 3])
-def test_advect4d_ystream_paths_match_oracle(vlp, orc, monkeypatch, n, force, nbuf):
+@pytest.mark.parametrize("tvx", [0, 4])
+def test_advect4d_ystream_paths_match_oracle(vlp, orc, monkeypatch, n, force, nbuf, tvx):
     monkeypatch.setenv("VLP4D_YS_FORCE", str(force))
     monkeypatch.setenv("VLP4D_YS_NBUF", str(nbuf))
+    if tvx:
+        monkeypatch.setenv("VLP4D_YS_TVX", str(tvx))
     steps = 2
     dt = 0.5 * (4 * math.pi / n[0]) / 6.0
     gc, oc = _configs(vlp, orc, n, dt=dt, nbiter=steps)
@@ -179,6 +183,8 @@ def test_advect4d_ystream_paths_match_oracle(vlp, orc, monkeypatch, n, force, nb
         f = sim.get_f()
         d = sim.diags()
     assert bits & 128, "geometry expected to take the y-streaming kernel"
+    if tvx:
+        assert (bits >> 8) & 15 == tvx
     assert relmax(f, st.f_interior()) < 1e-12
     for k in ("nrj", "nrjx", "nrjy", "l2norm"):
         assert relmax(d[k], st.diags()[k]) < 1e-12, k
