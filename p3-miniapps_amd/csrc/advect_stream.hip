@@ -57,25 +57,6 @@ template <typename A, int J> __device__ __forceinline__ A bspline_eta_j(A w) {
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// s_waitcnt vmcnt(n) for a wave-uniform n (the count is an immediate of the
-// instruction); n above 63 waits for all.
-#define VLP_VMWAIT(N) \
-  case N: __builtin_amdgcn_s_waitcnt(0x0F70 | ((N) & 15) | (((N) >> 4) << 14)); break;
-__device__ __forceinline__ void wait_vm(int n) {
-  switch (n) {
-    VLP_VMWAIT(1) VLP_VMWAIT(2) VLP_VMWAIT(3) VLP_VMWAIT(4) VLP_VMWAIT(5) VLP_VMWAIT(6) VLP_VMWAIT(7)
-    VLP_VMWAIT(8) VLP_VMWAIT(9) VLP_VMWAIT(10) VLP_VMWAIT(11) VLP_VMWAIT(12) VLP_VMWAIT(13) VLP_VMWAIT(14)
-    VLP_VMWAIT(15) VLP_VMWAIT(16) VLP_VMWAIT(17) VLP_VMWAIT(18) VLP_VMWAIT(19) VLP_VMWAIT(20) VLP_VMWAIT(21)
-    VLP_VMWAIT(22) VLP_VMWAIT(23) VLP_VMWAIT(24) VLP_VMWAIT(25) VLP_VMWAIT(26) VLP_VMWAIT(27) VLP_VMWAIT(28)
-    VLP_VMWAIT(29) VLP_VMWAIT(30) VLP_VMWAIT(31) VLP_VMWAIT(32) VLP_VMWAIT(33) VLP_VMWAIT(34) VLP_VMWAIT(35)
-    VLP_VMWAIT(36) VLP_VMWAIT(37) VLP_VMWAIT(38) VLP_VMWAIT(39) VLP_VMWAIT(40) VLP_VMWAIT(41) VLP_VMWAIT(42)
-    VLP_VMWAIT(43) VLP_VMWAIT(44) VLP_VMWAIT(45) VLP_VMWAIT(46) VLP_VMWAIT(47) VLP_VMWAIT(48) VLP_VMWAIT(49)
-    VLP_VMWAIT(50) VLP_VMWAIT(51) VLP_VMWAIT(52) VLP_VMWAIT(53) VLP_VMWAIT(54) VLP_VMWAIT(55) VLP_VMWAIT(56)
-    VLP_VMWAIT(57) VLP_VMWAIT(58) VLP_VMWAIT(59) VLP_VMWAIT(60) VLP_VMWAIT(61) VLP_VMWAIT(62) VLP_VMWAIT(63)
-    default: __builtin_amdgcn_s_waitcnt(0x0F70); break;
-  }
-}
-#undef VLP_VMWAIT
 
 // Per-workgroup constants and per-lane state of the streaming loop.  The slab
 // is kept in the storage type T (moved by LDS-DMA, 16 bytes per lane) and
