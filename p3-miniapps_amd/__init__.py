@@ -224,7 +224,9 @@ class Vlp4d:
         self._check(self.lib.vlp_save(self.h, os.fsencode(directory)))
 
     def kernel_variants(self):
-        """bit 0: scan x-spline, bit 1: register-line y/vx/vy splines, bit 2: blocked 4-D advection."""
+        """Which kernels this context runs (bit layout in include/vlp4d.h at
+        vlp_kernel_variants): e.g. bit 7 the y-streaming 4-D advection, bits
+        8-11 its vx tile, bit 12 the fused x/y spline + 2-D advection."""
         return self.lib.vlp_kernel_variants(self.h)
 
     def timers(self):
