@@ -358,13 +358,36 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   }
 };
 
+// Workgroup -> (vx tile, vy).  Workgroups go to the 8 XCDs (each with its
+// own L2) round-robin: block b runs on XCD b mod 8, as the (b / 8)-th of
+// that XCD.  The workgroups that stage the same C4 rows are those of the 5
+// neighbouring vy and the neighbouring vx tiles; with remap, XCD x takes the
+// vx tiles [x ntile/8, (x+1) ntile/8) for every vy, vy slower and tile
+// faster, so that the workgroups resident on one XCD at a time cover a
+// compact block of (tile, vy) and share their rows through its L2, while
+// every XCD gets its share of each vy (the vy = 0 row, whose feet mostly take
+// the direct gather, is the slowest).  Without remap (or when the tiles do
+// not split over the XCDs) block b is (b mod ntile, b / ntile).
+__device__ __forceinline__ void ys_block_tile(int b, int grid, int ntile, int N3, int remap, int &tile, int &ivy) {
+  constexpr int NX = 8;
+  if (remap && ntile % NX == 0 && grid == ntile * N3) {
+    const int x = b % NX, k = b / NX;
+    const int tx = ntile / NX;  // tiles per XCD
+    ivy = k / tx;
+    tile = x * tx + k % tx;
+  } else {
+    tile = b % ntile;
+    ivy = b / ntile;
+  }
+}
+
 }  // namespace
 
 template <typename T, typename A, int TVX, int NB>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(A) == 4 ? 3 : 1)))
 void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geom g,
                                                      FeetConsts fc, double dt, T *__restrict__ fout,
-                                                     int *__restrict__ err, int force) {
+                                                     int *__restrict__ err, int force, int remap) {
   extern __shared__ __align__(16) unsigned char ys_smem[];
   YStream<T, A, TVX, NB> s;
   s.force = force;
@@ -389,8 +412,9 @@ void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geo
   s.eb = (double *)(ys_smem + NB * (size_t)s.slab_bytes);
   s.nsteps = s.N1 + 3;
   const int ntile = s.N2 / TVX;
-  const int tile = blockIdx.x % ntile;
-  s.ivy = blockIdx.x / ntile;
+  int tile, ivy;
+  ys_block_tile(blockIdx.x, gridDim.x, ntile, s.N3, remap, tile, ivy);
+  s.ivy = ivy;
   const int tid = threadIdx.x;
   s.active = tid < s.N0 * TVX;
   s.ix = s.active ? tid % s.N0 : 0;
@@ -525,8 +549,10 @@ void launch_ys_t(const Geom &g, const YsPlan &p, const T *c4, const double *exy,
   const unsigned grid = (unsigned)((g.n[2] / TVX) * g.n[3]);
   int force = 0;  // test switch: VLP4D_YS_FORCE=1 rows per cell, 2 direct gather, 3 both
   if (const char *e = getenv("VLP4D_YS_FORCE")) force = atoi(e) & 3;
+  int remap = 1;  // VLP4D_YS_REMAP=0: block b -> (b mod ntile, b / ntile)
+  if (const char *e = getenv("VLP4D_YS_REMAP")) remap = atoi(e) != 0;
   hipLaunchKernelGGL((k_advect4d_ys<T, A, TVX, NB>), dim3(grid), dim3(p.block), p.lds, s, c4, exy, g,
-                     feet_consts(g), dt, fout, err, force);
+                     feet_consts(g), dt, fout, err, force, remap);
 }
 
 template <typename T, typename A, int TVX>

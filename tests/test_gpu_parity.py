@@ -162,8 +162,9 @@ def test_advect4d_variants_match_oracle(vlp, orc, monkeypatch, n, L, elds, remap
 # oracle's result, with two LDS plane buffers (the next plane's loads overlap
 # the step) and with three (loads two steps ahead), at the default vx tile
 # and at tile 4.  Shapes: a vy = 0 slab (Nvy even), Nx a multiple of 16 but
-# not of 64, several waves per workgroup and one.
-@pytest.mark.parametrize("n", [(32, 12, 20, 14), (48, 20, 16, 18), (64, 16, 32, 12)])
+# not of 64, several waves per workgroup and one; the last has 8 vx tiles, so
+# its workgroups take the XCD-grouped (tile, vy) order.
+@pytest.mark.parametrize("n", [(32, 12, 20, 14), (48, 20, 16, 18), (64, 16, 32, 12), (32, 12, 32, 16)])
 @pytest.mark.parametrize("force", [0, 1, 2, 3])
 @pytest.mark.parametrize("nbuf", [2, 3])
 @pytest.mark.parametrize("tvx", [0, 4])
