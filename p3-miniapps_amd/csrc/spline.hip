@@ -269,15 +269,14 @@ __global__ __launch_bounds__(256) void k_spline_line_scan(const T *__restrict__ 
 // 1); i1 and the outer index o select the tile's line family.  When the input
 // is contiguous across i1 as well (is1 == X0) the tiles run over the combined
 // index i0 + X0 i1 and each tile column finds its output position once.
-constexpr int TILE_XT = 64;
 typedef __attribute__((address_space(3))) void lds_void_t;
 
-template <typename T, int C>
+template <typename T, int C, int XT>
 __global__ __launch_bounds__(512) void k_spline_tile(const T *__restrict__ in, T *__restrict__ out, LineMap m, int P,
                                                      SplineConsts sc, double aC, double aN3) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *tile = reinterpret_cast<T *>(smem_raw);  // [N+4][XT]
-  constexpr int XT = TILE_XT, VPL = 16 / (int)sizeof(T), CPR = XT / VPL;  // 16-byte chunks per row
+  constexpr int VPL = 16 / (int)sizeof(T), CPR = XT / VPL;  // 16-byte chunks per row
   const int N = P * C;
   const bool comb = m.is1 == m.X0;
   const long XC = comb ? m.X0 * m.X1 : m.X0;  // contiguous input extent
@@ -380,7 +379,7 @@ static void launch_line_scan(const T *in, T *out, const LineMap &m, int P, int l
 #define VLP_LINE_CASE(CC) \
   case CC: return launch_line_scan<T, CC>(in, out, m, P, logP, s);
 
-template <typename T, int C>
+template <typename T, int C, int XT>
 static void launch_tile(const T *in, T *out, const LineMap &m, int P, hipStream_t s) {
   const SplineConsts sc = spline_consts();
   double aC = 1.0, aN3 = 1.0;
@@ -388,24 +387,38 @@ static void launch_tile(const T *in, T *out, const LineMap &m, int P, hipStream_
   for (int i = 0; i < P * C + 3; i++) aN3 *= sc.alpha;
   const bool comb = m.is1 == m.X0;
   const long XC = comb ? m.X0 * m.X1 : m.X0;
-  const long ntile = (XC + TILE_XT - 1) / TILE_XT;
-  const size_t lds = (size_t)(P * C + 4) * TILE_XT * sizeof(T);
-  const void *fn = (const void *)k_spline_tile<T, C>;
+  const long ntile = (XC + XT - 1) / XT;
+  const size_t lds = (size_t)(P * C + 4) * XT * sizeof(T);
+  const void *fn = (const void *)k_spline_tile<T, C, XT>;
   if (lds > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((k_spline_tile<T, C>), dim3((unsigned)(ntile * (comb ? 1 : m.X1) * m.A)), dim3(TILE_XT * P), lds,
+  hipLaunchKernelGGL((k_spline_tile<T, C, XT>), dim3((unsigned)(ntile * (comb ? 1 : m.X1) * m.A)), dim3(XT * P), lds,
                      s, in, out, m, P, sc, aC, aN3);
 }
 
 // Tile kernel when the contiguous index is i0 alone and the line splits into
-// P * C with C in [10, 16], P * 64 <= 512 lanes, the tile in LDS.
-static int tile_chunk(const LineMap &m, size_t tsz, int &P) {
+// P * C with C in [10, 16] and 64-position tiles (P * 64 <= 512 lanes), or --
+// wide tiles, N a multiple of 32 -- C = 32 and 128-position tiles (P * 128 <=
+// 512), the tile in LDS.  Wide tiles move twice the bytes per page a tile
+// touches: each tile row sits on its own memory page when the line stride is
+// large, and the address translations per byte halve.
+static int tile_chunk(const LineMap &m, size_t tsz, int &P, int &XT) {
   if (const char *e = getenv("VLP4D_SPLINE_TILE"))
     if (atoi(e) == 0) return 0;
   if (m.is0 != 1 || m.os0 != 1) return 0;
+  int wide = 1;  // VLP4D_SPLINE_WIDE=0: 64-position tiles only
+  if (const char *e = getenv("VLP4D_SPLINE_WIDE")) wide = atoi(e);
+  if (wide && m.N % 32 == 0 && (m.N / 32) * 128 <= 512 && ((m.N / 32) & (m.N / 32 - 1)) == 0 &&
+      (size_t)(m.N + 4) * 128 * tsz <= 160 * 1024 && (m.ils * tsz) % 16 == 0 && (m.is1 * tsz) % 16 == 0 &&
+      (m.ios * tsz) % 16 == 0 && ((m.is1 == m.X0 ? m.X0 * m.X1 : m.X0) * (long)tsz) % 16 == 0) {
+    P = m.N / 32;
+    XT = 128;
+    return 32;
+  }
+  XT = 64;
   int logP, C;
   x_split(m.N, P, logP, C);
-  if (!C || P * TILE_XT > 512) return 0;
-  if ((size_t)(m.N + 4) * TILE_XT * tsz > 160 * 1024) return 0;
+  if (!C || P * 64 > 512) return 0;
+  if ((size_t)(m.N + 4) * 64 * tsz > 160 * 1024) return 0;
   // input rows start 16-byte aligned (LDS-DMA of whole 16-byte chunks; the
   // buffers themselves are 256-byte aligned)
   if ((m.ils * tsz) % 16 || (m.is1 * tsz) % 16 || (m.ios * tsz) % 16) return 0;
@@ -417,15 +430,16 @@ static int tile_chunk(const LineMap &m, size_t tsz, int &P) {
 template <typename T> void launch_spline_line(const T *in, T *out, const LineMap &m, hipStream_t s) {
   int P, logP, C;
   {
-    int Pt;
-    switch (tile_chunk(m, sizeof(T), Pt)) {
-      case 10: return launch_tile<T, 10>(in, out, m, Pt, s);
-      case 11: return launch_tile<T, 11>(in, out, m, Pt, s);
-      case 12: return launch_tile<T, 12>(in, out, m, Pt, s);
-      case 13: return launch_tile<T, 13>(in, out, m, Pt, s);
-      case 14: return launch_tile<T, 14>(in, out, m, Pt, s);
-      case 15: return launch_tile<T, 15>(in, out, m, Pt, s);
-      case 16: return launch_tile<T, 16>(in, out, m, Pt, s);
+    int Pt, XT;
+    switch (tile_chunk(m, sizeof(T), Pt, XT)) {
+      case 10: return launch_tile<T, 10, 64>(in, out, m, Pt, s);
+      case 11: return launch_tile<T, 11, 64>(in, out, m, Pt, s);
+      case 12: return launch_tile<T, 12, 64>(in, out, m, Pt, s);
+      case 13: return launch_tile<T, 13, 64>(in, out, m, Pt, s);
+      case 14: return launch_tile<T, 14, 64>(in, out, m, Pt, s);
+      case 15: return launch_tile<T, 15, 64>(in, out, m, Pt, s);
+      case 16: return launch_tile<T, 16, 64>(in, out, m, Pt, s);
+      case 32: return launch_tile<T, 32, 128>(in, out, m, Pt, s);
       default: break;
     }
   }
