@@ -121,13 +121,17 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   __device__ __forceinline__ int fetch(int q, int b) {
     const int p = q + oyw - 1;  // y-position of the plane
     const char *src = (const char *)c4 + ((long)cbase + (long)F0 * (p + 3)) * (long)sizeof(T);
-    char *sb = (char *)slab + b * slab_bytes;
+    // the slot bookkeeping is recomputed each step from these (a few scalar
+    // operations) rather than kept live across the loop, where it would be
+    // spilled from the scalar registers
+    int ns = nslot, lb = b * slab_bytes + 1024 * wave, ls = 1024 * nwave;
+    asm volatile("" : "+s"(ns), "+s"(lb), "+s"(ls));
+    char *sb = (char *)slab + lb;
     int n = 0;
 #pragma unroll
     for (int k = 0; k < MS; k++)
-      if (k < nslot) {
-        __builtin_amdgcn_global_load_lds((const void *)(src + voff[k]), (lds_void *)(sb + 1024 * (wave + k * nwave)),
-                                         16, 0, 0);
+      if (k < ns) {
+        __builtin_amdgcn_global_load_lds((const void *)(src + voff[k]), (lds_void *)(sb + k * ls), 16, 0, 0);
         n++;
       }
     if (q < N1) {
