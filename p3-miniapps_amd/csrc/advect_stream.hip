@@ -85,6 +85,8 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   T *__restrict__ fout;
   T *slab;         // NB buffers of slab_bytes
   double *eb;      // [NB][3][EPR][2]
+  double *kconst;  // LDS: domain constants of the foot computation (KC_* layout)
+  static constexpr int KC_MIN = 0, KC_INV = 4, KC_LMIN = 8, KC_LMAX = 12, KC_VHI = 16, KC_N = 18;
   int slab_bytes;  // whole 1 KB blocks
   int EPR, cpr;    // E points per LDS row (whole 64-point chunks), chunks per row
   unsigned voff[MS];  // C4 byte offset of this lane's chunk in the wave's k-th slab DMA
@@ -189,6 +191,35 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
     c.acc = 0;
     c.key = -1;
     if (q >= N1) return;
+    // with double arithmetic the domain constants come from LDS (ds_read,
+    // wave-uniform address) at each open rather than living in scalar
+    // registers for the whole loop, where they would be spilled; float
+    // arithmetic (vector registers held to three waves per SIMD) keeps them
+    // in scalar registers
+    double minp[4], invd[4], lmin[4], lmax[4], vhi[2];
+    if constexpr (WY) {
+      const double *kc = kconst;
+      asm volatile("" : "+s"(kc));
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        minp[j] = kc[KC_MIN + j];
+        invd[j] = kc[KC_INV + j];
+        lmin[j] = kc[KC_LMIN + j];
+        lmax[j] = kc[KC_LMAX + j];
+      }
+      vhi[0] = kc[KC_VHI];
+      vhi[1] = kc[KC_VHI + 1];
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        minp[j] = g.minPhy[j];
+        invd[j] = g.inv_dx[j];
+        lmin[j] = fc.locmin[j];
+        lmax[j] = fc.locmax[j];
+      }
+      vhi[0] = g.minPhy[2] + fc.rxwidth[2];
+      vhi[1] = g.minPhy[3] + fc.rxwidth[3];
+    }
     const double ftmp1 = cy[0] * gx[0] + cy[1] * gx[1] + cy[2] * gx[2];
     const double ftmp2 = cy[0] * gy[0] + cy[1] * gy[1] + cy[2] * gy[2];
     const double xt2 = vx - 0.5 * dt * ftmp1;
@@ -197,24 +228,24 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
     xs[0] = x - dt * xt2;
     xs[1] = yq - dt * xt3;
     {
-      const double a = vx - dt * ftmp1, hi = g.minPhy[2] + fc.rxwidth[2], lo = g.minPhy[2];
+      const double a = vx - dt * ftmp1, hi = vhi[0], lo = minp[2];
       const double m = (hi < a) ? hi : a;
       xs[2] = (m < lo) ? lo : m;
     }
     {
-      const double a = vy - dt * ftmp2, hi = g.minPhy[3] + fc.rxwidth[3], lo = g.minPhy[3];
+      const double a = vy - dt * ftmp2, hi = vhi[1], lo = minp[3];
       const double m = (hi < a) ? hi : a;
       xs[3] = (m < lo) ? lo : m;
     }
     // all eight comparisons, no short cut (NaN fails them)
     bool ok = true;
 #pragma unroll
-    for (int j = 0; j < 4; j++) ok = ok & (xs[j] >= fc.locmin[j]) & (xs[j] <= fc.locmax[j]);
+    for (int j = 0; j < 4; j++) ok = ok & (xs[j] >= lmin[j]) & (xs[j] <= lmax[j]);
     int ip[4];
     A wyk[4];  // y weights, kept only for a direct gather
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      const double fj = g.inv_dx[j] * (xs[j] - g.minPhy[j]);
+      const double fj = invd[j] * (xs[j] - minp[j]);
       ip[j] = ok ? (int)floor(fj) : 0;
       if (j == 1) {
         c.wy = (A)(fj - ip[j]);
@@ -414,6 +445,20 @@ void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geo
   s.EPR = 64 * s.cpr;
   s.slab = (T *)ys_smem;
   s.eb = (double *)(ys_smem + NB * (size_t)s.slab_bytes);
+  s.kconst = s.eb + NB * 6 * s.EPR;
+  if (threadIdx.x == 0) {
+    double *k = s.kconst;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      k[s.KC_MIN + j] = g.minPhy[j];
+      k[s.KC_INV + j] = g.inv_dx[j];
+      k[s.KC_LMIN + j] = fc.locmin[j];
+      k[s.KC_LMAX + j] = fc.locmax[j];
+    }
+    k[s.KC_VHI] = g.minPhy[2] + fc.rxwidth[2];
+    k[s.KC_VHI + 1] = g.minPhy[3] + fc.rxwidth[3];
+  }
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the table is written before the first barrier
   s.nsteps = s.N1 + 3;
   const int ntile = s.N2 / TVX;
   int tile, ivy;
@@ -534,7 +579,8 @@ YsPlan ys_plan(const Geom &g, size_t tsz) {
     if ((N0 + 4) % (16 / (long)tsz) || N0 % 16) continue;  // whole 16-byte slab chunks per row
     const long ninstr = ys_slab_instr(N0, tvx, tsz);
     if ((ninstr + block / 64 - 1) / (block / 64) > 8) continue;  // DMA slots per wave
-    const size_t lds = nb * (size_t)ninstr * 1024 + nb * 3 * (size_t)ys_epr(N0) * 2 * sizeof(double);
+    const size_t lds = nb * (size_t)ninstr * 1024 + nb * 3 * (size_t)ys_epr(N0) * 2 * sizeof(double) +
+                       18 * sizeof(double);  // domain constants
     if (lds > 160 * 1024) continue;
     p.tvx = tvx;
     p.nb = nb;
