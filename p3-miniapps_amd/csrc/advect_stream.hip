@@ -30,30 +30,17 @@ namespace vlp {
 
 namespace {
 
-// One open cell: B-spline weights along x, y, vx, vy ([dim][k]; with double
-// arithmetic w[1] is unused and the y weight for the current plane is
-// evaluated from the fractional y position wy when needed), running sum, and
-// the packed stencil offsets
-// (ox+1) | (ovx+1) << 1 | (ovy+1) << 2, or -1 when the cell takes no streamed
-// contribution.
+// One open cell: B-spline weights along x, y, vx, vy ([dim][k]), running
+// sum, and the packed stencil offsets (ox+1) | (ovx+1) << 1 | (ovy+1) << 2,
+// or -1 when the cell takes no streamed contribution.  (Keeping the four y
+// weights rather than re-evaluating the one a plane needs from the
+// fractional y position measured 1 % faster with double arithmetic, at 246
+// registers.)
 template <typename A> struct OpenCell {
   A w[4][4];
-  A wy;
   A acc;
   int key;
 };
-
-// Weight j of the cubic B-spline at fractional position w: the same
-// expressions as bspline_eta (VMO/Advection.hpp:266-274), evaluated for one j.
-template <typename A, int J> __device__ __forceinline__ A bspline_eta_j(A w) {
-  const A e3 = (A)(1. / 6.) * w * w * w;
-  if (J == 3) return e3;
-  const A e0 = (A)(1. / 6.) + (A)0.5 * w * (w - (A)1.) - e3;
-  if (J == 0) return e0;
-  const A e2 = w + e0 - (A)2. * e3;
-  if (J == 2) return e2;
-  return (A)1. - e0 - e2 - e3;
-}
 
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -76,9 +63,8 @@ typedef __attribute__((address_space(3))) void lds_void;
 // and, as ip1 advances by one per step, computes only the new row.
 template <typename T, typename A, int TVX, int NB> struct YStream {
   static constexpr int VPL = 16 / (int)sizeof(T);  // slab values per lane load
-  // double arithmetic keeps each cell's fractional y position instead of its
-  // four y weights (16 registers fewer per lane); float keeps the weights
-  static constexpr bool WY = sizeof(A) == sizeof(double);
+  // double arithmetic reads the foot constants from LDS (see open())
+  static constexpr bool KLDS = sizeof(A) == sizeof(double);
   static constexpr int MS = 8;  // slab DMA instructions per wave per plane, at most
   const T *__restrict__ c4;
   const double *__restrict__ exy;  // (Ex, Ey) pairs, [N1][N0]
@@ -197,7 +183,7 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
     // arithmetic (vector registers held to three waves per SIMD) keeps them
     // in scalar registers
     double minp[4], invd[4], lmin[4], lmax[4], vhi[2];
-    if constexpr (WY) {
+    if constexpr (KLDS) {
       const double *kc = kconst;
       asm volatile("" : "+s"(kc));
 #pragma unroll
@@ -242,20 +228,11 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
 #pragma unroll
     for (int j = 0; j < 4; j++) ok = ok & (xs[j] >= lmin[j]) & (xs[j] <= lmax[j]);
     int ip[4];
-    A wyk[4];  // y weights, kept only for a direct gather
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const double fj = invd[j] * (xs[j] - minp[j]);
       ip[j] = ok ? (int)floor(fj) : 0;
-      if (j == 1) {
-        c.wy = (A)(fj - ip[j]);
-        bspline_eta<A>(c.wy, wyk);
-        if (!WY)
-#pragma unroll
-          for (int k = 0; k < 4; k++) c.w[1][k] = wyk[k];
-      } else {
-        bspline_eta<A>((A)(fj - ip[j]), c.w[j]);
-      }
+      bspline_eta<A>((A)(fj - ip[j]), c.w[j]);
     }
     const int ox = ip[0] - ix, oy = ip[1] - q, ovx = ip[2] - ivx, ovy = ip[3] - ivy;
     const bool fast = ok & ((unsigned)(ox + 1) < 2u) & (oy == oyw) & ((unsigned)(ovx + 1) < 2u) &
@@ -271,7 +248,7 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
 #pragma unroll
       for (int j = 0; j < 4; j++)
 #pragma unroll
-        for (int k = 0; k < 4; k++) e[j][k] = j == 1 ? wyk[k] : c.w[j][k];
+        for (int k = 0; k < 4; k++) e[j][k] = c.w[j][k];
       c.acc = gather4d<T, A>(c4, S1, S2, S3, ip, e);
     }
   }
@@ -353,7 +330,7 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   template <int R, int J> __device__ __forceinline__ void close_plane(const A t[4], unsigned sel) {
     constexpr int c = (R - J) & 3;
     OpenCell<A> &oc = cell[c];
-    const A add = (WY ? bspline_eta_j<A, J>(oc.wy) : oc.w[1][J]) * t[c];
+    const A add = oc.w[1][J] * t[c];
     oc.acc += ((sel >> c) & 1) ? add : (A)0;
   }
 
@@ -510,7 +487,6 @@ void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geo
   for (int c = 0; c < 4; c++) {
     s.cell[c].key = -1;
     s.cell[c].acc = 0;
-    s.cell[c].wy = 0;
 #pragma unroll
     for (int j = 0; j < 4; j++)
 #pragma unroll
