@@ -176,7 +176,12 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
     OpenCell<A> &c = cell[R];
     c.acc = 0;
     c.key = -1;
-    if (q >= N1) return;
+    if (q >= N1) {
+      // no cell: zero x-weights make its share of a common pass zero
+#pragma unroll
+      for (int k = 0; k < 4; k++) c.w[0][k] = 0;
+      return;
+    }
     // with double arithmetic the domain constants come from LDS (ds_read,
     // wave-uniform address) at each open rather than living in scalar
     // registers for the whole loop, where they would be spilled; float
@@ -250,6 +255,8 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
 #pragma unroll
         for (int k = 0; k < 4; k++) e[j][k] = c.w[j][k];
       c.acc = gather4d<T, A>(c4, S1, S2, S3, ip, e);
+#pragma unroll
+      for (int k = 0; k < 4; k++) c.w[0][k] = 0;  // no share of a common pass
     }
   }
 
@@ -259,65 +266,92 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
     return (tv + 1 + ovx) * RX + (1 + ovy) * (TVX + 4) * RX + ix + 1 + ox;
   }
 
+  // The plane's contribution t[c] to each of the four open cells from the 16
+  // rows at LDS element sp (one stencil key): rows (kvx, kvy) in order, the
+  // next row's LDS reads issued before the current row's FMAs (one row of
+  // look-ahead hides the LDS latency).
+  __device__ __forceinline__ void pass_rows(const T *sp, A t[4]) const {
+    const int VS = (TVX + 4) * RX;  // vy row stride
+    A u[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) t[c] = u[c] = 0;
+    T r[2][4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) r[0][k] = sp[k];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int kvx = i & 3, kvy = i >> 2;
+      // row i complete (only LDS reads are waited on; the slab DMA stays in
+      // flight), then row i+1's reads go out ahead of row i's FMAs
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+      if (i + 1 < 16) {
+        const T *rn = sp + ((i + 1) >> 2) * VS + ((i + 1) & 3) * RX;
+#pragma unroll
+        for (int k = 0; k < 4; k++) r[(i + 1) & 1][k] = rn[k];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      const A r0 = (A)r[i & 1][0], r1 = (A)r[i & 1][1], r2 = (A)r[i & 1][2], r3 = (A)r[i & 1][3];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const A *w0 = cell[c].w[0];
+        const A d = w0[0] * r0 + w0[1] * r1 + w0[2] * r2 + w0[3] * r3;
+        u[c] += cell[c].w[2][kvx] * d;
+      }
+      if (kvx == 3) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          t[c] += cell[c].w[3][kvy] * u[c];
+          u[c] = 0;
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+
   // Add plane q (LDS buffer b) to the four open cells; the cell opened at
-  // step q - j sits in slot (R - j) & 3 and takes y-weight j.  One pass reads
-  // the 16 rows of one stencil key per lane (the first key among its cells
-  // still to do) and adds to the cells with that key; the wave repeats until
-  // no lane has cells left (one pass when a lane's cells agree, the usual
-  // case).  force & 1 takes one cell per pass.
+  // step q - j sits in slot (R - j) & 3 and takes y-weight j.  When every
+  // lane's streamed cells agree on their stencil key (the usual case), one
+  // pass over the 16 rows of that key serves all four cells; cells without a
+  // streamed contribution have zero x-weights, so their share is zero.
+  // Otherwise one pass reads the rows of one key per lane (the first key
+  // among its cells still to do) and adds to the cells with that key; the
+  // wave repeats until no lane has cells left.  force & 1 takes the second
+  // form with one cell per pass.
   template <int R> __device__ __forceinline__ void accumulate(int b) {
     const T *s = (const T *)((const char *)slab + b * slab_bytes);
-    const int VS = (TVX + 4) * RX;  // vy row stride
+    int k0 = -1;
+    bool same = !(force & 1);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int k = cell[c].key;
+      if (k >= 0) {
+        if (k0 < 0) k0 = k;
+        else same = same && (k == k0);
+      }
+    }
+    A t[4];
+    if (__builtin_amdgcn_ballot_w64(!same) == 0) {
+      pass_rows(s + row_base(k0 < 0 ? 0 : k0), t);
+      close_plane<R, 0>(t, 15u);
+      close_plane<R, 1>(t, 15u);
+      close_plane<R, 2>(t, 15u);
+      close_plane<R, 3>(t, 15u);
+      return;
+    }
     unsigned todo = 0;
 #pragma unroll
     for (int c = 0; c < 4; c++) todo |= (cell[c].key >= 0 ? 1u : 0u) << c;
     while (__builtin_amdgcn_ballot_w64(todo != 0)) {
-      int k0 = 0;
+      int kp = 0;
 #pragma unroll
       for (int c = 3; c >= 0; c--)
-        if ((todo >> c) & 1) k0 = cell[c].key;
+        if ((todo >> c) & 1) kp = cell[c].key;
       unsigned sel = 0;
 #pragma unroll
       for (int c = 0; c < 4; c++)
-        if (((todo >> c) & 1) && cell[c].key == k0 && !((force & 1) && sel)) sel |= 1u << c;
+        if (((todo >> c) & 1) && cell[c].key == kp && !((force & 1) && sel)) sel |= 1u << c;
       todo &= ~sel;
-      A t[4], u[4];
-#pragma unroll
-      for (int c = 0; c < 4; c++) t[c] = u[c] = 0;
-      // rows (kvx, kvy) in order, the next row's LDS reads issued before the
-      // current row's FMAs (one row of look-ahead hides the LDS latency)
-      const T *sp = s + row_base(k0);
-      T r[2][4];
-#pragma unroll
-      for (int k = 0; k < 4; k++) r[0][k] = sp[k];
-#pragma unroll
-      for (int i = 0; i < 16; i++) {
-        const int kvx = i & 3, kvy = i >> 2;
-        // row i complete (only LDS reads are waited on; the slab DMA stays in
-        // flight), then row i+1's reads go out ahead of row i's FMAs
-        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-        if (i + 1 < 16) {
-          const T *rn = sp + ((i + 1) >> 2) * VS + ((i + 1) & 3) * RX;
-#pragma unroll
-          for (int k = 0; k < 4; k++) r[(i + 1) & 1][k] = rn[k];
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        const A r0 = (A)r[i & 1][0], r1 = (A)r[i & 1][1], r2 = (A)r[i & 1][2], r3 = (A)r[i & 1][3];
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-          const A *w0 = cell[c].w[0];
-          const A d = w0[0] * r0 + w0[1] * r1 + w0[2] * r2 + w0[3] * r3;
-          u[c] += cell[c].w[2][kvx] * d;
-        }
-        if (kvx == 3) {
-#pragma unroll
-          for (int c = 0; c < 4; c++) {
-            t[c] += cell[c].w[3][kvy] * u[c];
-            u[c] = 0;
-          }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-      }
+      pass_rows(s + row_base(kp), t);
       close_plane<R, 0>(t, sel);
       close_plane<R, 1>(t, sel);
       close_plane<R, 2>(t, sel);
@@ -326,12 +360,13 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   }
 
   // The cell opened J steps ago takes the plane's contribution with its y
-  // weight J, when this pass selected it.
+  // weight J, when sel has its bit.
   template <int R, int J> __device__ __forceinline__ void close_plane(const A t[4], unsigned sel) {
     constexpr int c = (R - J) & 3;
     OpenCell<A> &oc = cell[c];
     const A add = oc.w[1][J] * t[c];
-    oc.acc += ((sel >> c) & 1) ? add : (A)0;
+    if (sel == 15u) oc.acc += add;
+    else oc.acc += ((sel >> c) & 1) ? add : (A)0;
   }
 
   // Write cell y = q - 3 (slot (R + 1) & 3).
