@@ -165,6 +165,33 @@ int vlp_kernel_variants(const vlp_ctx *ctx);
  * ctx is NULL). */
 const char *vlp_last_error(const vlp_ctx *ctx);
 
+/* Autotuner of the 4-D advection's workgroup shape (vx tile, LDS plane
+ * buffers): every shape the geometry admits is timed on the context's own
+ * buffers and the fastest is used by its later steps; csv_path (optional)
+ * receives the timings.  Follows the reference's tile tuner, which scans the
+ * candidate tiles, keeps the minimum and logs a CSV
+ * (miniapps/vlp4d_mpi/kokkos/Tile_size_tuning.hpp:51-96,127-148; per-kernel
+ * registration miniapps/vlp4d_mpi/kokkos/Tuning.hpp:26-65).  The timed launches write F of the next
+ * step, which that step overwrites.  The VLP4D_YS_* environment overrides
+ * disable it. */
+int vlp_autotune(vlp_ctx *ctx, const char *csv_path);
+/* The tuned (vx tile, plane buffers), 0 and 0 before vlp_autotune. */
+int vlp_tuned_tile(const vlp_ctx *ctx, int *vx_tile, int *plane_buffers);
+
+/* Ensembles: independent contexts (any grid sizes and precisions, one
+ * device) advanced together, two steps of every member per launch of one
+ * captured HIP graph (their kernels overlap on the GPU); an odd remainder
+ * runs eagerly.  Same results as vlp_step on each member.  No reference
+ * counterpart (the reference runs one solve per process); for batched
+ * studies of many small solves.  Destroy the ensemble before its members;
+ * calls on the ensemble and on its members must not overlap. */
+typedef struct vlp_ensemble vlp_ensemble;
+int vlp_ensemble_create(vlp_ctx *const *members, int n, vlp_ensemble **out);
+/* Advance every member by nsteps (each member's iter + nsteps <= nbiter). */
+int vlp_ensemble_step(vlp_ensemble *ens, int nsteps);
+int vlp_ensemble_size(const vlp_ensemble *ens);
+int vlp_ensemble_destroy(vlp_ensemble *ens);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,7 +33,8 @@ EXPORTS = [
     "vlp_iter", "vlp_get_config", "vlp_diags", "vlp_save", "vlp_get_f", "vlp_set_f",
     "vlp_buffer_size", "vlp_get_buffer", "vlp_spline_xy", "vlp_advect2d_poisson",
     "vlp_spline_vxvy", "vlp_advect4d_poisson", "vlp_timer", "vlp_timer_label", "vlp_last_error",
-    "vlp_kernel_variants",
+    "vlp_kernel_variants", "vlp_autotune", "vlp_tuned_tile", "vlp_ensemble_create", "vlp_ensemble_step",
+    "vlp_ensemble_size", "vlp_ensemble_destroy",
 ]
 
 
@@ -112,6 +113,12 @@ def load(path=LIB_PATH):
     lib.vlp_last_error.argtypes = [vp]
     lib.vlp_last_error.restype = ctypes.c_char_p
     lib.vlp_kernel_variants.argtypes = [vp]
+    lib.vlp_autotune.argtypes = [vp, ctypes.c_char_p]
+    lib.vlp_tuned_tile.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_int)]
+    lib.vlp_ensemble_create.argtypes = [P(vp), ctypes.c_int, P(vp)]
+    lib.vlp_ensemble_step.argtypes = [vp, ctypes.c_int]
+    lib.vlp_ensemble_size.argtypes = [vp]
+    lib.vlp_ensemble_destroy.argtypes = [vp]
     _lib = lib
     return lib
 
@@ -229,6 +236,18 @@ class Vlp4d:
         8-11 its vx tile, bit 12 the fused x/y spline + 2-D advection."""
         return self.lib.vlp_kernel_variants(self.h)
 
+    def autotune(self, csv_path=None):
+        """Time every workgroup shape of the 4-D advection this geometry
+        admits on this context's buffers and keep the fastest; returns it
+        as (vx_tile, plane_buffers)."""
+        self._check(self.lib.vlp_autotune(self.h, os.fsencode(csv_path) if csv_path else None))
+        return self.tuned_tile()
+
+    def tuned_tile(self):
+        t, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.vlp_tuned_tile(self.h, ctypes.byref(t), ctypes.byref(b)))
+        return t.value, b.value
+
     def timers(self):
         out = {}
         for k, lab in enumerate(TIMER_LABELS):
@@ -237,3 +256,38 @@ class Vlp4d:
             self.lib.vlp_timer(self.h, k, ctypes.byref(s), ctypes.byref(c))
             out[lab] = (s.value, c.value)
         return out
+
+
+class Ensemble:
+    """Independent simulations (Vlp4d contexts on one device, any grids and
+    precisions) advanced together through one captured HIP graph per two
+    steps.  Close the ensemble before its members."""
+
+    def __init__(self, members):
+        self.lib = load()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h for m in self.members])
+        h = ctypes.c_void_p()
+        rc = self.lib.vlp_ensemble_create(arr, len(self.members), ctypes.byref(h))
+        if rc:
+            raise Vlp4dError(rc, self.lib.vlp_last_error(None).decode())
+        self.h = h
+
+    def step(self, nsteps=1):
+        rc = self.lib.vlp_ensemble_step(self.h, int(nsteps))
+        if rc:
+            raise Vlp4dError(rc, self.lib.vlp_last_error(None).decode())
+
+    def __len__(self):
+        return self.lib.vlp_ensemble_size(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.vlp_ensemble_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -332,8 +332,8 @@ static void launch_advect4d_a(const Geom &g, const T *c4, const double *ex, cons
 template <>
 bool launch_advect4d<double>(const Geom &g, const double *c4, const double *ex, const double *ey, const double *exy,
                              double dt, bool, double *fout, double *rho_part, double *l2_part, int *err,
-                             hipStream_t s) {
-  if (launch_advect4d_ystream<double, double>(g, c4, exy, dt, fout, err, s)) return true;
+                             hipStream_t s, int ys_tvx, int ys_nb) {
+  if (launch_advect4d_ystream<double, double>(g, c4, exy, dt, fout, err, s, ys_tvx, ys_nb)) return true;
   launch_advect4d_a<double, double>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
   return false;
 }
@@ -341,9 +341,9 @@ bool launch_advect4d<double>(const Geom &g, const double *c4, const double *ex, 
 template <>
 bool launch_advect4d<float>(const Geom &g, const float *c4, const double *ex, const double *ey, const double *exy,
                             double dt, bool arith32, float *fout, double *rho_part, double *l2_part, int *err,
-                            hipStream_t s) {
-  const bool ys = arith32 ? launch_advect4d_ystream<float, float>(g, c4, exy, dt, fout, err, s)
-                          : launch_advect4d_ystream<float, double>(g, c4, exy, dt, fout, err, s);
+                            hipStream_t s, int ys_tvx, int ys_nb) {
+  const bool ys = arith32 ? launch_advect4d_ystream<float, float>(g, c4, exy, dt, fout, err, s, ys_tvx, ys_nb)
+                          : launch_advect4d_ystream<float, double>(g, c4, exy, dt, fout, err, s, ys_tvx, ys_nb);
   if (ys) return true;
   if (arith32) launch_advect4d_a<float, float>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);
   else launch_advect4d_a<float, double>(g, c4, ex, ey, dt, fout, rho_part, l2_part, err, s);

@@ -565,12 +565,14 @@ struct YsPlan {
 long ys_slab_instr(long N0, long tvx, size_t tsz) { return (5 * (tvx + 4) * ((N0 + 4) * (long)tsz / 16) + 63) / 64; }
 long ys_epr(long N0) { return (N0 + 63) / 64 * 64; }
 
-YsPlan ys_plan(const Geom &g, size_t tsz) {
+// want_tvx / want_nb: a chosen vx tile and buffer count (0: free); the
+// environment (VLP4D_YS_TVX, VLP4D_YS_NBUF) overrides them.
+YsPlan ys_plan(const Geom &g, size_t tsz, int want_tvx = 0, int want_nb = 0) {
   YsPlan p;
   const long N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
   const long c4n = (N0 + 6) * (N1 + 6) * (N2 + 6) * (N3 + 6) + 64;
   if (N0 % 2 || c4n >= (1L << 31)) return p;
-  int force = 0, force_nb = 0;
+  int force = want_tvx, force_nb = want_nb;
   if (const char *e = getenv("VLP4D_YS_TVX")) force = atoi(e);
   if (const char *e = getenv("VLP4D_YS_NBUF")) force_nb = atoi(e);
   // Preference order measured at 128^4: two or more small workgroups per CU
@@ -629,16 +631,22 @@ void launch_ys_nb(const Geom &g, const YsPlan &p, const T *c4, const double *exy
 // streaming kernel.
 template <typename T, typename A>
 bool launch_advect4d_ystream(const Geom &g, const T *c4, const double *exy, double dt, T *fout, int *err,
-                             hipStream_t s) {
+                             hipStream_t s, int ys_tvx, int ys_nb) {
   if (const char *e = getenv("VLP4D_YSTREAM"))
     if (atoi(e) == 0) return false;
-  const YsPlan p = ys_plan(g, sizeof(T));
+  YsPlan p = ys_plan(g, sizeof(T), ys_tvx, ys_nb);
+  if (!p.tvx && (ys_tvx || ys_nb)) p = ys_plan(g, sizeof(T));  // a choice this geometry cannot take
   switch (p.tvx) {
     case 8: launch_ys_nb<T, A, 8>(g, p, c4, exy, dt, fout, err, s); return true;
     case 4: launch_ys_nb<T, A, 4>(g, p, c4, exy, dt, fout, err, s); return true;
     case 2: launch_ys_nb<T, A, 2>(g, p, c4, exy, dt, fout, err, s); return true;
     default: return false;
   }
+}
+
+bool ystream_eligible(const Geom &g, size_t tsz, int tvx, int nb) {
+  const YsPlan p = ys_plan(g, tsz, tvx, nb);
+  return p.tvx == tvx && p.nb == nb;
 }
 
 int ystream_tvx(const Geom &g, bool fp32_storage, bool arith32) {
@@ -649,10 +657,10 @@ int ystream_tvx(const Geom &g, bool fp32_storage, bool arith32) {
 }
 
 template bool launch_advect4d_ystream<double, double>(const Geom &, const double *, const double *, double, double *,
-                                                      int *, hipStream_t);
+                                                      int *, hipStream_t, int, int);
 template bool launch_advect4d_ystream<float, double>(const Geom &, const float *, const double *, double, float *,
-                                                     int *, hipStream_t);
+                                                     int *, hipStream_t, int, int);
 template bool launch_advect4d_ystream<float, float>(const Geom &, const float *, const double *, double, float *, int *,
-                                                    hipStream_t);
+                                                    hipStream_t, int, int);
 
 }  // namespace vlp

@@ -213,8 +213,10 @@ __global__ __launch_bounds__(256) void k_analytic_field(const double *__restrict
 }
 
 // Diags::compute and computeL2norm, VMO/Diags.cpp:16-76.
+// The record index is iter, or -- iter_dev non-null, for launches replayed
+// from a captured graph -- *iter_dev + 1, which is then stored back.
 __global__ __launch_bounds__(256) void k_diag_final(Geom g, FieldBufs b, const double *__restrict__ l2_part, long n_l2,
-                                                    int iter, int with_l2) {
+                                                    int iter, int with_l2, int *__restrict__ iter_dev) {
   __shared__ double red[8];
   const int N1 = g.n[1];
   double s[4];
@@ -230,7 +232,12 @@ __global__ __launch_bounds__(256) void k_diag_final(Geom g, FieldBufs b, const d
     l2 = block_sum(v, red);
   }
   if (threadIdx.x == 0) {
+    if (iter_dev) {
+      iter = *iter_dev + 1;
+      *iter_dev = iter;
+    }
     const int nb = b.nbiter + 1;
+    if (iter < 0 || iter >= nb) return;
     double it_nrj = sqrt(s[1] * g.dx[0] * g.dx[1]);
     it_nrj = it_nrj > 1.e-30 ? log(it_nrj) : -1.e9;
     b.diag[0 * nb + iter] = it_nrj;
@@ -242,7 +249,7 @@ __global__ __launch_bounds__(256) void k_diag_final(Geom g, FieldBufs b, const d
 }
 
 void launch_field(const Geom &g, const FieldBufs &b, const double *rho_part, long rho_outer, const double *l2_part,
-                  long n_l2, int iter, bool record_diag, hipStream_t s) {
+                  long n_l2, int iter, bool record_diag, hipStream_t s, int *iter_dev) {
   const int N0 = g.n[0], N1 = g.n[1];
   const int nxh = N0 / 2 + 1;
   if (rho_outer > 64 && b.rho_tmp) {
@@ -261,7 +268,8 @@ void launch_field(const Geom &g, const FieldBufs &b, const double *rho_part, lon
     hipLaunchKernelGGL(k_c2rx_diag, dim3(N1), dim3(256), sizeof(double) * (6 * nxh + 8), s, g, b);
   }
   if (record_diag)
-    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, g, b, l2_part, n_l2, iter, l2_part != nullptr ? 1 : 0);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, g, b, l2_part, n_l2, iter, l2_part != nullptr ? 1 : 0,
+                       iter_dev);
 }
 
 }  // namespace vlp

@@ -53,6 +53,7 @@ struct vlp_ctx {
   double tsec[VLP_T_COUNT] = {};
   int tcalls[VLP_T_COUNT] = {};
   hipEvent_t ev[8] = {};
+  int ys_tvx = 0, ys_nb = 0;  // 4-D advection tile chosen by vlp_autotune (0: preference order)
 };
 
 namespace {
@@ -208,7 +209,7 @@ template <typename T> void run_advect2d(vlp_ctx *c) {
 template <typename T> bool run_advect4d(vlp_ctx *c) {
   const bool need = vlp::launch_advect4d<T>(c->g, (const T *)c->c4, c->fb.ex, c->fb.ey, c->fb.exy, c->cfg.dt,
                                             c->precision == VLP_FP32_ACC, (T *)c->fnext, c->rho_part, c->l2_part,
-                                            c->d_err, c->stream);
+                                            c->d_err, c->stream, c->ys_tvx, c->ys_nb);
   c->rho_outer = c->g.n[3];
   c->half_rho = false;
   return need;
@@ -247,9 +248,9 @@ int sync_and_check(vlp_ctx *c) {
   return VLP_OK;
 }
 
-int field(vlp_ctx *c, int iter, bool record, bool with_l2) {
+int field(vlp_ctx *c, int iter, bool record, bool with_l2, int *iter_dev = nullptr) {
   vlp::launch_field(c->g, c->fb, c->rho_part, c->rho_outer, with_l2 ? c->l2_part : nullptr, c->n_l2, iter, record,
-                    c->stream);
+                    c->stream, iter_dev);
   return check_launch(c, "field");
 }
 
@@ -259,31 +260,44 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-// One onetimestep (VMO/Timestep.hpp:34-97) + swap, events between phases.
-int step_once(vlp_ctx *c) {
+// The kernels of one onetimestep (VMO/Timestep.hpp:34-97) on the context's
+// stream, then the swap of f and fnext.  Eager (iter_dev null): HIP events
+// between the phases, diagnostics at index iter + 1.  Under graph capture
+// (iter_dev non-null): no events, the diagnostics index comes from the device
+// counter *iter_dev (advanced by the field solve).
+int enqueue_step(vlp_ctx *c, int *iter_dev) {
   const bool f64 = c->precision == VLP_FP64;
+  const bool ev_on = iter_dev == nullptr;
   hipEvent_t *ev = c->ev;
-  HIP_TRY(c, hipEventRecord(ev[0], c->stream));
+  if (ev_on) HIP_TRY(c, hipEventRecord(ev[0], c->stream));
   if (f64) run_spline_xy<double>(c); else run_spline_xy<float>(c);
-  HIP_TRY(c, hipEventRecord(ev[1], c->stream));
+  if (ev_on) HIP_TRY(c, hipEventRecord(ev[1], c->stream));
   if (f64) run_advect2d<double>(c); else run_advect2d<float>(c);
-  HIP_TRY(c, hipEventRecord(ev[2], c->stream));
+  if (ev_on) HIP_TRY(c, hipEventRecord(ev[2], c->stream));
   int rc = field(c, c->iter + 1, false, false);
   if (rc) return rc;
-  HIP_TRY(c, hipEventRecord(ev[3], c->stream));
+  if (ev_on) HIP_TRY(c, hipEventRecord(ev[3], c->stream));
   if (f64) run_spline_vxvy<double>(c); else run_spline_vxvy<float>(c);
-  HIP_TRY(c, hipEventRecord(ev[4], c->stream));
+  if (ev_on) HIP_TRY(c, hipEventRecord(ev[4], c->stream));
   const bool moments = f64 ? run_advect4d<double>(c) : run_advect4d<float>(c);
-  HIP_TRY(c, hipEventRecord(ev[5], c->stream));
+  if (ev_on) HIP_TRY(c, hipEventRecord(ev[5], c->stream));
   if (moments) {
     if (f64) run_moments_next<double>(c); else run_moments_next<float>(c);
   }
-  rc = field(c, c->iter + 1, true, true);
+  rc = field(c, c->iter + 1, true, true, iter_dev);
   if (rc) return rc;
-  HIP_TRY(c, hipEventRecord(ev[6], c->stream));
+  if (ev_on) HIP_TRY(c, hipEventRecord(ev[6], c->stream));
   rc = check_launch(c, "step");
   if (rc) return rc;
   std::swap(c->f, c->fnext);
+  return VLP_OK;
+}
+
+// One onetimestep + swap, events between phases.
+int step_once(vlp_ctx *c) {
+  hipEvent_t *ev = c->ev;
+  int rc = enqueue_step(c, nullptr);
+  if (rc) return rc;
   c->iter++;
   rc = sync_and_check(c);
   if (rc) return rc;
@@ -677,7 +691,249 @@ const char *vlp_timer_label(int which) { return (which >= 0 && which < VLP_T_COU
 const char *vlp_last_error(const vlp_ctx *c) { return c ? c->err.c_str() : g_last_error.c_str(); }
 
 int vlp_kernel_variants(const vlp_ctx *c) {
-  return c ? vlp::kernel_variants(c->g, c->precision != VLP_FP64, c->precision == VLP_FP32_ACC) : -1;
+  if (!c) return -1;
+  int bits = vlp::kernel_variants(c->g, c->precision != VLP_FP64, c->precision == VLP_FP32_ACC);
+  if ((bits & 128) && c->ys_tvx) bits = (bits & ~(15 << 8)) | (c->ys_tvx << 8);
+  return bits;
+}
+
+// Autotuner of the 4-D advection's workgroup shape: every (vx tile, plane
+// buffers) this geometry admits is launched on the context's own buffers
+// (one warm-up and three timed launches, HIP events on its stream) and the
+// fastest is kept for the context's later steps.  Same scan-and-keep-the-
+// minimum scheme as the reference's tile tuner
+// (miniapps/vlp4d_mpi/kokkos/Tile_size_tuning.hpp:51-96), with its CSV log
+// (:127-148).  The launches write fnext, which
+// the next step overwrites, and report errors to a scratch flag.
+int vlp_autotune(vlp_ctx *c, const char *csv_path) {
+  int rc = usable(c);
+  if (rc) return rc;
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (getenv("VLP4D_YS_TVX") || getenv("VLP4D_YS_NBUF") || getenv("VLP4D_YSTREAM")) return VLP_OK;
+  const size_t tsz = c->precision == VLP_FP64 ? sizeof(double) : sizeof(float);
+  struct Cand { int tvx, nb; float ms; };
+  std::vector<Cand> cands;
+  for (int tvx : {2, 4, 8})
+    for (int nb : {2, 3})
+      if (vlp::ystream_eligible(c->g, tsz, tvx, nb)) cands.push_back({tvx, nb, 0.f});
+  if (cands.empty()) return VLP_OK;
+  int *d_scratch = nullptr;
+  HIP_TRY(c, hipMalloc((void **)&d_scratch, sizeof(int)));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+  for (auto &k : cands) {
+    if (!ok) break;
+    for (int it = 0; it < 4 && ok; it++) {
+      if (it == 1) ok = hipEventRecord(e0, c->stream) == hipSuccess;
+      if (c->precision == VLP_FP64)
+        vlp::launch_advect4d_ystream<double, double>(c->g, (const double *)c->c4, c->fb.exy, c->cfg.dt,
+                                                     (double *)c->fnext, d_scratch, c->stream, k.tvx, k.nb);
+      else if (c->precision == VLP_FP32)
+        vlp::launch_advect4d_ystream<float, double>(c->g, (const float *)c->c4, c->fb.exy, c->cfg.dt,
+                                                    (float *)c->fnext, d_scratch, c->stream, k.tvx, k.nb);
+      else
+        vlp::launch_advect4d_ystream<float, float>(c->g, (const float *)c->c4, c->fb.exy, c->cfg.dt,
+                                                   (float *)c->fnext, d_scratch, c->stream, k.tvx, k.nb);
+      ok = ok && hipGetLastError() == hipSuccess;
+    }
+    ok = ok && hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+    float ms = 0.f;
+    ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    k.ms = ms / 3.f;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(d_scratch);
+  if (!ok) return set_err(c, VLP_E_HIP, "autotune launch failed");
+  const Cand *best = &cands[0];
+  for (const auto &k : cands)
+    if (k.ms < best->ms) best = &k;
+  c->ys_tvx = best->tvx;
+  c->ys_nb = best->nb;
+  if (csv_path && *csv_path) {
+    FILE *f = fopen(csv_path, "w");
+    if (!f) return set_err(c, VLP_E_FILE, std::string("cannot write ") + csv_path);
+    fprintf(f, "kernel,nx,ny,nvx,nvy,precision,vx_tile,plane_buffers,ms_per_launch,chosen\n");
+    for (const auto &k : cands)
+      fprintf(f, "advect4d,%d,%d,%d,%d,%d,%d,%d,%.6f,%d\n", c->g.n[0], c->g.n[1], c->g.n[2], c->g.n[3], c->precision,
+              k.tvx, k.nb, k.ms, &k == best ? 1 : 0);
+    fclose(f);
+  }
+  return VLP_OK;
+}
+
+int vlp_tuned_tile(const vlp_ctx *c, int *vx_tile, int *plane_buffers) {
+  if (!c || !vx_tile || !plane_buffers) return set_err(nullptr, VLP_E_ARG, "null argument");
+  *vx_tile = c->ys_tvx;
+  *plane_buffers = c->ys_nb;
+  return VLP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ensembles: independent contexts (any grids and precisions, one device)
+// advanced together.  Two steps of every member are captured once into one
+// HIP graph -- a fork event on the ensemble's stream, each member's kernels
+// on its own stream, a join -- so that the members' kernels overlap on the
+// GPU and a pair of steps of the whole ensemble is a single graph launch.
+// Two steps, because each step ends by swapping f and fnext on the host: a
+// pair returns every member to its buffers, and the graph stays valid.  The
+// diagnostics index of each member comes from a device counter; error flags
+// are sticky and are read once after the replays.
+// ---------------------------------------------------------------------------
+struct vlp_ensemble {
+  std::vector<vlp_ctx *> m;
+  int device = 0;
+  hipStream_t root = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  int *d_iter = nullptr;  // one counter per member
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<void *> cap_f;  // each member's f when the graph was captured
+  double seconds = 0.;
+  long steps = 0;
+};
+
+namespace {
+
+void ensemble_free(vlp_ensemble *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (auto ev : e->join)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  if (e->d_iter) (void)hipFree(e->d_iter);
+  if (e->root) (void)hipStreamDestroy(e->root);
+  e->exec = nullptr;
+  e->graph = nullptr;
+}
+
+int ensemble_capture(vlp_ensemble *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+  if (hipStreamBeginCapture(e->root, hipStreamCaptureModeRelaxed) != hipSuccess)
+    return set_err(nullptr, VLP_E_HIP, "hipStreamBeginCapture failed");
+  int rc = VLP_OK;
+  if (hipEventRecord(e->fork, e->root) != hipSuccess) rc = set_err(nullptr, VLP_E_HIP, "fork event");
+  for (size_t i = 0; i < e->m.size() && !rc; i++)
+    if (hipStreamWaitEvent(e->m[i]->stream, e->fork, 0) != hipSuccess) rc = set_err(nullptr, VLP_E_HIP, "fork wait");
+  for (size_t i = 0; i < e->m.size() && !rc; i++) {
+    rc = enqueue_step(e->m[i], e->d_iter + i);
+    if (!rc) rc = enqueue_step(e->m[i], e->d_iter + i);
+  }
+  for (size_t i = 0; i < e->m.size() && !rc; i++)
+    if (hipEventRecord(e->join[i], e->m[i]->stream) != hipSuccess ||
+        hipStreamWaitEvent(e->root, e->join[i], 0) != hipSuccess)
+      rc = set_err(nullptr, VLP_E_HIP, "join");
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(e->root, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ee != hipSuccess || !g) return set_err(nullptr, VLP_E_HIP, "hipStreamEndCapture failed");
+  e->graph = g;
+  if (hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0) != hipSuccess)
+    return set_err(nullptr, VLP_E_HIP, "hipGraphInstantiate failed");
+  e->cap_f.resize(e->m.size());
+  for (size_t i = 0; i < e->m.size(); i++) e->cap_f[i] = e->m[i]->f;
+  return VLP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vlp_ensemble_create(vlp_ctx *const *members, int n, vlp_ensemble **out) {
+  if (!out) return set_err(nullptr, VLP_E_ARG, "null output pointer");
+  *out = nullptr;
+  if (!members || n < 1) return set_err(nullptr, VLP_E_ARG, "an ensemble needs at least one member");
+  for (int i = 0; i < n; i++) {
+    if (!members[i]) return set_err(nullptr, VLP_E_ARG, "null member");
+    if (members[i]->broken) return set_err(nullptr, VLP_E_ARG, "a member is in an error state");
+    if (members[i]->device != members[0]->device) return set_err(nullptr, VLP_E_ARG, "members on different devices");
+    for (int j = 0; j < i; j++)
+      if (members[j] == members[i]) return set_err(nullptr, VLP_E_ARG, "a context appears twice");
+  }
+  auto *e = new vlp_ensemble();
+  e->m.assign(members, members + n);
+  e->device = members[0]->device;
+  e->join.assign(n, nullptr);
+  bool ok = hipSetDevice(e->device) == hipSuccess &&
+            hipStreamCreateWithFlags(&e->root, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess &&
+            hipMalloc((void **)&e->d_iter, n * sizeof(int)) == hipSuccess;
+  for (int i = 0; i < n && ok; i++) ok = hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    ensemble_free(e);
+    delete e;
+    return set_err(nullptr, VLP_E_HIP, "ensemble set-up failed");
+  }
+  *out = e;
+  return VLP_OK;
+}
+
+int vlp_ensemble_step(vlp_ensemble *e, int nsteps) {
+  if (!e) return set_err(nullptr, VLP_E_ARG, "null ensemble");
+  if (nsteps < 0) return set_err(nullptr, VLP_E_ARG, "negative step count");
+  for (auto *c : e->m) {
+    if (c->broken) return set_err(nullptr, VLP_E_ARG, "a member is in an error state");
+    if (c->iter + nsteps > c->cfg.nbiter) return set_err(nullptr, VLP_E_ARG, "step count exceeds a member's nbiter");
+  }
+  if (hipSetDevice(e->device) != hipSuccess) return set_err(nullptr, VLP_E_HIP, "hipSetDevice failed");
+  const auto t0 = std::chrono::steady_clock::now();
+  const int pairs = nsteps / 2;
+  if (pairs > 0) {
+    for (auto *c : e->m)
+      if (hipStreamSynchronize(c->stream) != hipSuccess) return set_err(nullptr, VLP_E_HIP, "member stream");
+    bool fresh = e->exec != nullptr;
+    for (size_t i = 0; i < e->m.size() && fresh; i++) fresh = e->cap_f[i] == e->m[i]->f;
+    if (!fresh) {
+      const int rc = ensemble_capture(e);
+      if (rc) return rc;
+    }
+    std::vector<int> it(e->m.size());
+    for (size_t i = 0; i < e->m.size(); i++) it[i] = e->m[i]->iter;
+    if (hipMemcpy(e->d_iter, it.data(), it.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return set_err(nullptr, VLP_E_HIP, "iteration counters");
+    for (int k = 0; k < pairs; k++)
+      if (hipGraphLaunch(e->exec, e->root) != hipSuccess) return set_err(nullptr, VLP_E_HIP, "hipGraphLaunch failed");
+    if (hipStreamSynchronize(e->root) != hipSuccess) return set_err(nullptr, VLP_E_HIP, "ensemble replay failed");
+    for (auto *c : e->m) {
+      c->iter += 2 * pairs;
+      const int rc = sync_and_check(c);
+      if (rc) return set_err(nullptr, rc, c->err);
+    }
+  }
+  if (nsteps % 2)
+    for (auto *c : e->m) {
+      const int rc = step_once(c);
+      if (rc) return set_err(nullptr, rc, c->err);
+    }
+  const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  e->seconds += dt;
+  e->steps += nsteps;
+  for (auto *c : e->m) {
+    c->tsec[VLP_T_MAINLOOP] += dt;
+    c->tsec[VLP_T_TOTAL] += dt;
+    c->tcalls[VLP_T_MAINLOOP] += nsteps;
+  }
+  return VLP_OK;
+}
+
+int vlp_ensemble_size(const vlp_ensemble *e) { return e ? (int)e->m.size() : -1; }
+
+int vlp_ensemble_destroy(vlp_ensemble *e) {
+  if (!e) return VLP_OK;
+  (void)hipSetDevice(e->device);
+  if (e->root) (void)hipStreamSynchronize(e->root);
+  ensemble_free(e);
+  delete e;
+  return VLP_OK;
 }
 
 }  // extern "C"

@@ -52,7 +52,8 @@ void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, bool arith
 // (launch_moments); false when the kernel produced them.
 template <typename T>
 bool launch_advect4d(const Geom &g, const T *c4, const double *ex, const double *ey, const double *exy, double dt,
-                     bool arith32, T *fout, double *rho_part, double *l2_part, int *err, hipStream_t s);
+                     bool arith32, T *fout, double *rho_part, double *l2_part, int *err, hipStream_t s,
+                     int ys_tvx = 0, int ys_nb = 0);
 // Fused spline x/y + 2-D half-step advection + density partials (xy_fused.hip):
 // f -> CXY and rho_part[grp][vy][y][x] for grp < G.  Returns false without
 // launching when the geometry does not fit; xy_fused_groups gives G (0: the
@@ -64,8 +65,13 @@ int xy_fused_groups(const Geom &g, size_t tsz);
 // y-streaming 4-D advection (f only; rho and L2 by launch_moments afterwards).
 // Returns false without launching when the geometry does not fit it.
 template <typename T, typename A>
+// ys_tvx / ys_nb (0: the preference order) select the vx tile and LDS plane
+// buffers; the environment overrides both.
 bool launch_advect4d_ystream(const Geom &g, const T *c4, const double *exy, double dt, T *fout, int *err,
-                             hipStream_t s);
+                             hipStream_t s, int ys_tvx = 0, int ys_nb = 0);
+// Whether the streaming kernel takes (tvx, nb) for this geometry and storage
+// size (the autotuner's candidate filter).
+bool ystream_eligible(const Geom &g, size_t tsz, int tvx, int nb);
 // vx tile of the y-streaming kernel for a geometry (0: not used).
 int ystream_tvx(const Geom &g, bool fp32_storage, bool arith32);
 template <typename T>
@@ -102,7 +108,9 @@ struct FieldBufs {
 // G*N3 after the fused xy kernel).  Beyond 64 planes they are first summed in
 // RHO_SLICES slices by a wide kernel.
 constexpr int RHO_SLICES = 16;
+// iter_dev (optional): device-side iteration counter for graph replays; the
+// diagnostics then go to index *iter_dev + 1, which is stored back.
 void launch_field(const Geom &g, const FieldBufs &b, const double *rho_part, long rho_outer, const double *l2_part,
-                  long n_l2, int iter, bool record_diag, hipStream_t s);
+                  long n_l2, int iter, bool record_diag, hipStream_t s, int *iter_dev = nullptr);
 
 }  // namespace vlp
