@@ -143,6 +143,64 @@ def cpu_baseline(n, steps=1):
                       f"{steps} step(s) at {n}^4, {sec:.2f} s"}
 
 
+def ensemble_bench(pkg, steps, seed=20261202):
+    """Batched ensemble of independent Landau solves (one captured HIP graph
+    per two steps of every member): 16 at 32^4, 16 at 48^2 x 64^2, 16 at
+    64^4, f stored in float with float interpolation arithmetic, perturbation
+    amplitude alpha ~ U[0.01, 0.5] and wave number k in {0.3, 0.5} from a
+    seeded generator, box [0, 2 pi / k)^2 x [-6, 6]^2, dt = dx / (2 V),
+    autotuned 4-D advection tiles.  Reports solves per second (a solve = the
+    given number of steps of one member) and the same members stepped one
+    after another for comparison."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    shapes = [(32, 32, 32, 32)] * 16 + [(48, 48, 64, 64)] * 16 + [(64, 64, 64, 64)] * 16
+    members, cfgs = [], []
+    for n in shapes:
+        alpha = float(rng.uniform(0.01, 0.5))
+        k = float(rng.choice([0.3, 0.5]))
+        lx, vmax = 2 * math.pi / k, 6.0
+        dt = 0.5 * (lx / max(n[0], n[1])) / vmax
+        cfg = pkg.VlpConfig.make(list(n), [0.0, 0.0, -vmax, -vmax], [lx, lx, vmax, vmax], 10, dt, 2 * steps + 2)
+        sim = pkg.Vlp4d(cfg, precision=pkg.FP32_ACC)
+        x = np.arange(n[0]) * (lx / n[0])
+        y = np.arange(n[1]) * (lx / n[1])
+        vx = -vmax + np.arange(n[2]) * (2 * vmax / n[2])
+        vy = -vmax + np.arange(n[3]) * (2 * vmax / n[3])
+        sp = 1 + alpha * np.cos(k * x)[None, :] * np.cos(k * y)[:, None]           # [y][x]
+        mv = np.exp(-0.5 * (vx[None, :] ** 2 + vy[:, None] ** 2)) / (2 * math.pi)  # [vy][vx]
+        sim.set_f(mv[:, :, None, None] * sp[None, None, :, :])
+        sim.autotune()
+        members.append(sim)
+        cfgs.append((n, alpha, k))
+    # the same members stepped one after another (eager), then the ensemble
+    t0 = time.perf_counter()
+    for m in members:
+        m.step(steps)
+    seq = time.perf_counter() - t0
+    with pkg.Ensemble(members) as ens:
+        ens.step(2)  # capture + first replay
+        t0 = time.perf_counter()
+        ens.step(steps)
+        par = time.perf_counter() - t0
+    tiles = sorted({(c[0], m.tuned_tile()) for c, m in zip(cfgs, members)})
+    cells = sum(int(np.prod(n)) for n in shapes)
+    for m in members:
+        m.close()
+    return {
+        "metric": f"ensemble solves/s ({len(shapes)} Landau solves of {steps} steps, uneven grids, FP32 f)",
+        "value": len(shapes) / par, "unit": "solves/s", "n_gpus": 1, "steps": steps, "warmup": 2,
+        "ms_per_step": 1e3 * par / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-storage, f32 interpolation arithmetic",
+        "data": f"synthetic: Landau initial conditions, alpha ~ U[0.01, 0.5], k in {{0.3, 0.5}}, seed {seed}",
+        "config": {"workload": "batched ensemble: 16 x 32^4, 16 x 48^2 x 64^2, 16 x 64^4, one HIP graph per two steps",
+                   "members": len(shapes), "cells": cells, "tuned_tiles": [[list(t[0]), list(t[1])] for t in tiles]},
+        "cell_updates_per_s": cells * steps / par,
+        "sequential_solves_per_s": len(shapes) / seq,
+        "speedup_vs_sequential": seq / par,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +212,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="batched-ensemble workload (48 uneven Landau solves); --steps is the solve length")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -163,6 +223,9 @@ def main():
         calibrate_copy()
         return
     pkg = load_pkg()
+    if a.ensemble:
+        print(json.dumps(ensemble_bench(pkg, a.steps)))
+        return
     n = a.n
     nb = a.warmup + a.steps
     cfg = pkg.VlpConfig.make(n, SLD10["minphy"], SLD10["maxphy"], SLD10["idcase"], SLD10["dt"], nb)
