@@ -178,6 +178,20 @@ int vlp_autotune(vlp_ctx *ctx, const char *csv_path);
 /* The tuned (vx tile, plane buffers), 0 and 0 before vlp_autotune. */
 int vlp_tuned_tile(const vlp_ctx *ctx, int *vx_tile, int *plane_buffers);
 
+/* Adjoint of one step at the context's current f (VLP_FP64 contexts):
+ * grad = (d f^{n+1} / d f^n)^T g, g and grad interior arrays of n = Nx*Ny*
+ * Nvx*Nvy doubles (x fastest), the derivative taken through the spline
+ * coefficients, the 4-D interpolation, the characteristic feet and the field
+ * of the half step.  Where a velocity foot sits exactly on the box bound
+ * (the clamp of VMO/Advection.hpp:214-217 has a corner there) the branch in
+ * force is differentiated.  Contributions are summed with atomic adds, so
+ * repeated calls agree to round-off, not bit for bit.  The context is not
+ * advanced; its coefficient, density and field buffers are left holding the
+ * intermediates of the recomputed step.
+ * It has no counterpart in the reference, which has no derivative code; it
+ * returns VLP_E_CFL where the step from this f would. */
+int vlp_adjoint_step(vlp_ctx *ctx, const double *g, double *grad, size_t n);
+
 /* Ensembles: independent contexts (any grid sizes and precisions, one
  * device) advanced together, two steps of every member per launch of one
  * captured HIP graph (their kernels overlap on the GPU); an odd remainder

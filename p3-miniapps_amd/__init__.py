@@ -34,7 +34,7 @@ EXPORTS = [
     "vlp_buffer_size", "vlp_get_buffer", "vlp_spline_xy", "vlp_advect2d_poisson",
     "vlp_spline_vxvy", "vlp_advect4d_poisson", "vlp_timer", "vlp_timer_label", "vlp_last_error",
     "vlp_kernel_variants", "vlp_autotune", "vlp_tuned_tile", "vlp_ensemble_create", "vlp_ensemble_step",
-    "vlp_ensemble_size", "vlp_ensemble_destroy",
+    "vlp_ensemble_size", "vlp_ensemble_destroy", "vlp_adjoint_step",
 ]
 
 
@@ -119,6 +119,7 @@ def load(path=LIB_PATH):
     lib.vlp_ensemble_step.argtypes = [vp, ctypes.c_int]
     lib.vlp_ensemble_size.argtypes = [vp]
     lib.vlp_ensemble_destroy.argtypes = [vp]
+    lib.vlp_adjoint_step.argtypes = [vp, dp, dp, ctypes.c_size_t]
     _lib = lib
     return lib
 
@@ -206,6 +207,15 @@ class Vlp4d:
     def set_f(self, f):
         a = np.ascontiguousarray(f, dtype=np.float64).ravel()
         self._check(self.lib.vlp_set_f(self.h, _dptr(a), a.size))
+
+    def adjoint_step(self, g):
+        """Vector-Jacobian product of one step at the current f (FP64
+        contexts): (d f^{n+1} / d f^n)^T g, same shape as get_f().  The
+        context is not advanced."""
+        a = np.ascontiguousarray(g, dtype=np.float64).ravel()
+        out = np.empty_like(a)
+        self._check(self.lib.vlp_adjoint_step(self.h, _dptr(a), _dptr(out), a.size))
+        return out.reshape(self.shape4[::-1])
 
     def buffer(self, which):
         n = self.lib.vlp_buffer_size(self.h, which)

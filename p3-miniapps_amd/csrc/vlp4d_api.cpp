@@ -54,6 +54,10 @@ struct vlp_ctx {
   int tcalls[VLP_T_COUNT] = {};
   hipEvent_t ev[8] = {};
   int ys_tvx = 0, ys_nb = 0;  // 4-D advection tile chosen by vlp_autotune (0: preference order)
+  // adjoint of one step (allocated on first use)
+  double *adj_c4s = nullptr, *adj_blk = nullptr, *adj_tmp = nullptr, *adj_cxy = nullptr;
+  double *adj_E = nullptr, *adj_rho = nullptr, *adj_k = nullptr, *adj_g = nullptr, *adj_grad = nullptr;
+  double *adj_Mt[4] = {};  // M_d^T of the line spline operator, N_d x (N_d + 4)
 };
 
 namespace {
@@ -129,7 +133,9 @@ std::vector<double> twiddles(int n) {
 void free_all(vlp_ctx *c) {
   void *bufs[] = {c->f,        c->fnext,    c->cxy,       c->tv,     c->c4,      c->rho_part, c->l2_part,
                   c->fb.rho,   c->fb.ex,    c->fb.ey,     c->fb.exy, c->fb.spec1, c->fb.spec2, c->fb.twx,  c->fb.twy,
-                  c->fb.diag_part, c->fb.rho_tmp, c->fb.diag, c->d_err};
+                  c->fb.diag_part, c->fb.rho_tmp, c->fb.diag, c->d_err, c->adj_c4s, c->adj_blk, c->adj_tmp,
+                  c->adj_cxy, c->adj_E, c->adj_rho, c->adj_k, c->adj_g, c->adj_grad, c->adj_Mt[0], c->adj_Mt[1],
+                  c->adj_Mt[2], c->adj_Mt[3]};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_err) (void)hipHostFree(c->h_err);
@@ -760,6 +766,105 @@ int vlp_autotune(vlp_ctx *c, const char *csv_path) {
     fclose(f);
   }
   return VLP_OK;
+}
+
+// Adjoint of one step at the context's current f (FP64): grad = (d f^{n+1} /
+// d f^n)^T g, without advancing the context.  The forward phases are
+// recomputed from f (spline_xy, half step, field, spline_vxvy); the adjoint
+// is then C4^T I4^T g plus the foot term through E, K^T and the half step
+// (adjoint.hip).  The coefficient, density and field buffers are left
+// holding intermediate values; the next step recomputes them from f.
+int vlp_adjoint_step(vlp_ctx *c, const double *g_host, double *grad_host, size_t n) {
+  int rc = usable(c);
+  if (rc) return rc;
+  if (c->precision != VLP_FP64) return set_err(c, VLP_E_ARG, "the adjoint is implemented for VLP_FP64 contexts");
+  if (!g_host || !grad_host || n != c->n_f) return set_err(c, VLP_E_SIZE, "f buffer size mismatch");
+  const Geom &g = c->g;
+  const long N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const long E0 = N0 + 4, E1 = N1 + 4, E2 = N2 + 4, E3 = N3 + 4;
+  const size_t nblk = (size_t)E0 * E1 * E2 * E3;
+  hipStream_t s = c->stream;
+#define ADJ_ALLOC(p, bytes)                                                         \
+  if (!(p) && hipMalloc((void **)&(p), (bytes)) != hipSuccess)                       \
+    return set_err(c, VLP_E_HIP, "hipMalloc failed (adjoint buffers)");
+  ADJ_ALLOC(c->adj_c4s, c->n_c4s * sizeof(double));
+  ADJ_ALLOC(c->adj_blk, nblk * sizeof(double));
+  ADJ_ALLOC(c->adj_tmp, nblk * sizeof(double));
+  ADJ_ALLOC(c->adj_cxy, c->n_cxy * sizeof(double));
+  ADJ_ALLOC(c->adj_E, 2 * N0 * N1 * sizeof(double));
+  ADJ_ALLOC(c->adj_rho, N0 * N1 * sizeof(double));
+  ADJ_ALLOC(c->adj_g, c->n_f * sizeof(double));
+  ADJ_ALLOC(c->adj_grad, c->n_f * sizeof(double));
+  const bool first_mt = c->adj_Mt[0] == nullptr;
+  for (int d = 0; d < 4; d++) ADJ_ALLOC(c->adj_Mt[d], (size_t)g.n[d] * (g.n[d] + 4) * sizeof(double));
+#undef ADJ_ALLOC
+  if (first_mt) {
+    // M_d from the product spline kernel on unit vectors: line j = e_j ->
+    // coefficients M[.][j], stored as Mt[j][m] (row j of M^T)
+    for (int d = 0; d < 4; d++) {
+      const int N = g.n[d];
+      std::vector<double> id((size_t)N * N, 0.);
+      for (int j = 0; j < N; j++) id[(size_t)j * N + j] = 1.;
+      double *d_id = nullptr;
+      HIP_TRY(c, hipMalloc((void **)&d_id, id.size() * sizeof(double)));
+      HIP_TRY(c, hipMemcpy(d_id, id.data(), id.size() * sizeof(double), hipMemcpyHostToDevice));
+      vlp::LineMap m{};
+      m.N = N;
+      m.X0 = N; m.X1 = 1; m.A = 1;
+      m.is0 = N; m.is1 = 0; m.ios = 0; m.ils = 1;
+      m.os0 = N + 4; m.os1 = 0; m.oos = 0; m.ols = 1;
+      vlp::launch_spline_line<double>(d_id, c->adj_Mt[d], m, s);
+      const hipError_t e = hipStreamSynchronize(s);
+      (void)hipFree(d_id);
+      if (e != hipSuccess) return set_err(c, VLP_E_HIP, "line operator");
+    }
+  }
+  // idcases 2 and 6 set an analytic E that does not depend on f (VMO/Field.cpp:81-108)
+  const bool field_term = g.idcase != 2 && g.idcase != 6;
+  if (field_term && !c->adj_k) {
+    // kx, ky: E of a unit density at the origin (the field solve drops the
+    // mean, so its constant shift does not enter)
+    if (hipMalloc((void **)&c->adj_k, 2 * N0 * N1 * sizeof(double)) != hipSuccess)
+      return set_err(c, VLP_E_HIP, "hipMalloc failed (adjoint buffers)");
+    HIP_TRY(c, hipMemsetAsync(c->rho_part, 0, N0 * N1 * sizeof(double), s));
+    const double unit = 1. / (g.dx[2] * g.dx[3]);
+    HIP_TRY(c, hipMemcpyAsync(c->rho_part, &unit, sizeof(double), hipMemcpyHostToDevice, s));
+    c->rho_outer = 1;
+    rc = field(c, c->iter, false, false);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(c->adj_k, c->fb.ex, N0 * N1 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->adj_k + N0 * N1, c->fb.ey, N0 * N1 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+  }
+  // forward phases of the step from f (f itself untouched)
+  run_spline_xy<double>(c);
+  run_advect2d<double>(c);
+  rc = field(c, c->iter + 1, false, false);
+  if (rc) return rc;
+  run_spline_vxvy<double>(c);
+  HIP_TRY(c, hipMemcpyAsync(c->adj_g, g_host, c->n_f * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemsetAsync(c->adj_c4s, 0, c->n_c4s * sizeof(double), s));
+  HIP_TRY(c, hipMemsetAsync(c->adj_E, 0, 2 * N0 * N1 * sizeof(double), s));
+  vlp::launch_adj_advect4d(g, (const double *)c->c4, c->fb.ex, c->fb.ey, c->cfg.dt, c->adj_g, c->adj_c4s, c->adj_E,
+                           field_term, s);
+  // direct term: (M_x^T M_y^T M_vx^T M_vy^T) of the coefficient adjoint
+  vlp::launch_c4_block(g, c->adj_c4s, c->adj_blk, s);
+  vlp::launch_axis_contract(c->adj_Mt[3], (int)E3, (int)N3, c->adj_blk, c->adj_tmp, 1, E0 * E1 * E2, false, s);
+  vlp::launch_axis_contract(c->adj_Mt[2], (int)E2, (int)N2, c->adj_tmp, c->adj_blk, N3, E0 * E1, false, s);
+  vlp::launch_axis_contract(c->adj_Mt[1], (int)E1, (int)N1, c->adj_blk, c->adj_tmp, N2 * N3, E0, false, s);
+  vlp::launch_axis_contract(c->adj_Mt[0], (int)E0, (int)N0, c->adj_tmp, c->adj_grad, N1 * N2 * N3, 1, false, s);
+  if (field_term) {
+    // through E: K^T, then the half step and C_xy^T
+    vlp::launch_poisson_T(c->adj_E, c->adj_k, c->adj_k + N0 * N1, c->adj_rho, (int)N0, (int)N1, s);
+    HIP_TRY(c, hipMemsetAsync(c->adj_cxy, 0, c->n_cxy * sizeof(double), s));
+    vlp::launch_adj_advect2d(c->adj_rho, g, 0.5 * c->cfg.dt, c->adj_cxy, s);
+    vlp::launch_axis_contract(c->adj_Mt[1], (int)E1, (int)N1, c->adj_cxy, c->adj_tmp, N2 * N3, E0, false, s);
+    vlp::launch_axis_contract(c->adj_Mt[0], (int)E0, (int)N0, c->adj_tmp, c->adj_grad, N1 * N2 * N3, 1, true, s);
+  }
+  rc = check_launch(c, "adjoint");
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(grad_host, c->adj_grad, c->n_f * sizeof(double), hipMemcpyDeviceToHost, s));
+  return sync_and_check(c);  // the recomputed phases flag a foot out of the box
 }
 
 int vlp_tuned_tile(const vlp_ctx *c, int *vx_tile, int *plane_buffers) {

@@ -90,6 +90,16 @@ long l2_partials(const Geom &g);
 // advection kernel replaces the x, y and advect2d kernels.
 int kernel_variants(const Geom &g, bool fp32_storage, bool arith32);
 
+// Adjoint of one step (adjoint.hip), FP64.
+void launch_adj_advect4d(const Geom &g, const double *c4, const double *ex, const double *ey, double dt,
+                         const double *gin, double *lc4, double *lE, bool field_term, hipStream_t s);
+void launch_axis_contract(const double *Mt, int Min, int Nout, const double *in, double *out, long O, long I,
+                          bool accumulate, hipStream_t s);
+void launch_c4_block(const Geom &g, const double *c4s, double *blk, hipStream_t s);
+void launch_poisson_T(const double *lE, const double *kx, const double *ky, double *lrho, int N0, int N1,
+                      hipStream_t s);
+void launch_adj_advect2d(const double *lrho, const Geom &g, double dth, double *lcxy, hipStream_t s);
+
 // Field solve (FP64): rho from the per-vy partial sums, Poisson by DFT with
 // the reference's FFT conventions, field diagnostics; optional L2 norm.
 struct FieldBufs {
