@@ -91,6 +91,23 @@ def load():
     lib.vo_save.argtypes = [vp, ctypes.c_char_p]
     lib.vo_get_interior.argtypes = [C, dp, dp]
     lib.vo_set_interior.argtypes = [C, dp, dp]
+    lib.vl_lag_basis.argtypes = [ctypes.c_double, dp]
+    lib.vl_advect_xy.argtypes = [C, ctypes.c_int, dp, dp, ctypes.c_double]
+    lib.vl_advect_v.argtypes = [C, ctypes.c_int, dp, dp, dp, ctypes.c_double]
+    lib.vl_field_rho.argtypes = [C, dp, dp]
+    lib.vl_init_case.argtypes = [C, dp]
+    lib.vl_create.argtypes = [C, P(vp)]
+    lib.vl_step.argtypes = [vp]
+    lib.vl_run.argtypes = [vp, ctypes.c_int]
+    lib.vl_destroy.argtypes = [vp]
+    lib.vl_iter.argtypes = [vp]
+    lib.vl_f.argtypes = [vp]
+    lib.vl_f.restype = dp
+    lib.vl_set_f.argtypes = [vp, dp]
+    lib.vl_field.argtypes = [vp, ctypes.c_int]
+    lib.vl_field.restype = dp
+    lib.vl_diag_series.argtypes = [vp, ctypes.c_int]
+    lib.vl_diag_series.restype = dp
     _lib = lib
     return lib
 
@@ -279,6 +296,98 @@ class State:
     def close(self):
         if self.h:
             self.lib.vo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- Lagrange split-step vlp4d (miniapps/vlp4d/openmp) ----------------------
+
+def lag_basis(px):
+    out = np.zeros(6)
+    load().vl_lag_basis(px, _p(out))
+    return out
+
+
+def lag_advect_xy(c, d, f, dt):
+    """One 1-D advection along x (d=0) or y (d=1) by dt*v; f has no halo."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.empty_like(f)
+    load().vl_advect_xy(ctypes.byref(c), d, _p(f), _p(out), dt)
+    return out
+
+
+def lag_advect_v(c, d, f, e, dt):
+    """One 1-D advection along vx (d=2) or vy (d=3) by dt*E(x, y)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    out = np.empty_like(f)
+    load().vl_advect_v(ctypes.byref(c), d, _p(f), _p(out), _p(e), dt)
+    return out
+
+
+def lag_field_rho(c, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    rho = np.zeros((c.nxmax[1], c.nxmax[0]))
+    load().vl_field_rho(ctypes.byref(c), _p(f), _p(rho))
+    return rho
+
+
+def lag_init_case(c):
+    f = np.zeros(interior_shape(c))
+    rc = load().vl_init_case(ctypes.byref(c), _p(f))
+    if rc:
+        raise RuntimeError(f"vl_init_case failed ({rc})")
+    return f
+
+
+class LagState:
+    """vl_state: the Lagrange split-step driver (init + onetimestep)."""
+
+    def __init__(self, c):
+        self.lib = load()
+        self.c = c
+        h = ctypes.c_void_p()
+        rc = self.lib.vl_create(ctypes.byref(c), ctypes.byref(h))
+        if rc:
+            raise RuntimeError(f"vl_create failed ({rc})")
+        self.h = h
+
+    def step(self, n=1):
+        rc = self.lib.vl_run(self.h, n)
+        if rc:
+            raise RuntimeError(f"vl_run failed ({rc})")
+
+    @property
+    def iter(self):
+        return self.lib.vl_iter(self.h)
+
+    def f(self):
+        n = int(np.prod(interior_shape(self.c)))
+        return np.ctypeslib.as_array(self.lib.vl_f(self.h), shape=(n,)).reshape(interior_shape(self.c)).copy()
+
+    def set_f(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.size == int(np.prod(interior_shape(self.c)))
+        self.lib.vl_set_f(self.h, _p(f))
+
+    def field(self, which):
+        n = self.c.nxmax[0] * self.c.nxmax[1]
+        p = self.lib.vl_field(self.h, which)
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(self.c.nxmax[1], self.c.nxmax[0]).copy()
+
+    def diags(self):
+        n = self.c.nbiter + 1
+        return {name: np.ctypeslib.as_array(self.lib.vl_diag_series(self.h, k), shape=(n,)).copy()
+                for k, name in enumerate(["nrj", "mass"])}
+
+    def close(self):
+        if self.h:
+            self.lib.vl_destroy(self.h)
             self.h = None
 
     def __del__(self):

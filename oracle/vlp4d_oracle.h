@@ -112,6 +112,25 @@ int vo_save(const vo_state *st, const char *dir);            /* VMO/Diags.cpp:78
 void vo_get_interior(const vo_config *cfg, const double *fn, double *out);
 void vo_set_interior(const vo_config *cfg, double *fn, const double *in);
 
+/* ---- Lagrange split-step vlp4d (miniapps/vlp4d/openmp, "VO/"), vlp4d_lag.c ----
+ * Arrays without halo: N0*N1*N2*N3 doubles, x fastest. */
+typedef struct vl_state vl_state;
+void vl_lag_basis(double px, double coef[6]);                                     /* VO/advect.hpp:72-86 */
+void vl_advect_xy(const vo_config *cfg, int d, const double *fn, double *fnp1, double dt);   /* :112-204, d 0|1 */
+void vl_advect_v(const vo_config *cfg, int d, const double *fn, double *fnp1, const double *e,
+                 double dt);                                                      /* :206-301, d 2|3 */
+void vl_field_rho(const vo_config *cfg, const double *fn, double *rho);           /* VO/field.cpp:5-29 */
+int vl_init_case(const vo_config *cfg, double *fn);                               /* VO/init.cpp:144-285 */
+int vl_create(const vo_config *cfg, vl_state **out);   /* init + field + diag 0, VO/vlp4d.cpp:55-58 */
+int vl_step(vl_state *st);                             /* onetimestep, VO/timestep.hpp:11-55 */
+int vl_run(vl_state *st, int nsteps);
+void vl_destroy(vl_state *st);
+int vl_iter(const vl_state *st);
+const double *vl_f(const vl_state *st);
+void vl_set_f(vl_state *st, const double *f);
+const double *vl_field(const vl_state *st, int which);        /* 0 phi, 1 ex, 2 ey (last solve) */
+const double *vl_diag_series(const vl_state *st, int which);  /* 0 nrj, 1 mass (nbiter+1) */
+
 #ifdef __cplusplus
 }
 #endif
