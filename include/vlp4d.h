@@ -48,6 +48,10 @@ enum {
  * in every mode. */
 enum { VLP_FP64 = 64, VLP_FP32 = 32, VLP_FP32_ACC = 33 };
 
+/* Solvers: the spline semi-Lagrangian step of miniapps/vlp4d_mpi (default) or
+ * the Lagrange split-step of miniapps/vlp4d (vlp_create_lagrange). */
+enum { VLP_SCHEME_SPLINE = 0, VLP_SCHEME_LAGRANGE = 1 };
+
 /* Domain of VMO/Config.hpp:34-48 for one rank (dx derived as (max-min)/N). */
 typedef struct {
   int nxmax[4];       /* points in x, y, vx, vy                         */
@@ -94,6 +98,16 @@ int vlp_parse_deck(const char *dat_path, vlp_config *cfg);
  * VLP_FP32 or VLP_FP32_ACC.  dat_path variant parses the deck first. */
 int vlp_create(const char *dat_path, int precision, int device, vlp_ctx **out);
 int vlp_create_config(const vlp_config *cfg, int precision, int device, vlp_ctx **out);
+/* The Lagrange split-step solver of the non-MPI miniapp, FP64: init() and
+ * the initial field solve of miniapps/vlp4d/openmp/init.cpp:287-302 and
+ * vlp4d.cpp:55-58 (its own TSI20 profile, init.cpp:256-285).  vlp_step then
+ * runs its onetimestep (timestep.hpp:11-55: 1-D order-5 Lagrange advections
+ * x/2, y/2, field solve with that step's diagnostics, vx, vy, x/2, y/2);
+ * vlp_diags fills nrj and mass (diags.cpp:13-39; nrjx/nrjy are computed as
+ * well, l2norm stays 0), vlp_save writes its 3-column nrj.out
+ * (diags.cpp:41-52).  The spline-only calls (phases, vlp_autotune,
+ * vlp_adjoint_step, ensembles) return VLP_E_ARG on such a context. */
+int vlp_create_lagrange(const vlp_config *cfg, int device, vlp_ctx **out);
 
 /* finalize() of VMO/Init.cpp:375-382 (without the file output; see vlp_save). */
 int vlp_destroy(vlp_ctx *ctx);

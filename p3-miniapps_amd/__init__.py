@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(HERE, "libvlp4d.so")
 
 OK, E_FILE, E_CASE, E_CFL, E_ARG, E_HIP, E_SIZE = range(7)
 FP64, FP32, FP32_ACC = 64, 32, 33
+SCHEME_SPLINE, SCHEME_LAGRANGE = 0, 1
 BUF_F, BUF_CXY, BUF_C4, BUF_RHO, BUF_EX, BUF_EY = range(6)
 TIMER_LABELS = ["total", "MainLoop", "pack", "comm", "unpack", "advec2D", "advec4D", "field",
                 "all_reduce", "Fourier", "diag", "splinecoeff_xy", "splinecoeff_vxvy"]
@@ -34,7 +35,7 @@ EXPORTS = [
     "vlp_buffer_size", "vlp_get_buffer", "vlp_spline_xy", "vlp_advect2d_poisson",
     "vlp_spline_vxvy", "vlp_advect4d_poisson", "vlp_timer", "vlp_timer_label", "vlp_last_error",
     "vlp_kernel_variants", "vlp_autotune", "vlp_tuned_tile", "vlp_ensemble_create", "vlp_ensemble_step",
-    "vlp_ensemble_size", "vlp_ensemble_destroy", "vlp_adjoint_step",
+    "vlp_ensemble_size", "vlp_ensemble_destroy", "vlp_adjoint_step", "vlp_create_lagrange",
 ]
 
 
@@ -120,6 +121,7 @@ def load(path=LIB_PATH):
     lib.vlp_ensemble_size.argtypes = [vp]
     lib.vlp_ensemble_destroy.argtypes = [vp]
     lib.vlp_adjoint_step.argtypes = [vp, dp, dp, ctypes.c_size_t]
+    lib.vlp_create_lagrange.argtypes = [P(VlpConfig), ctypes.c_int, P(vp)]
     _lib = lib
     return lib
 
@@ -141,10 +143,18 @@ def parse_deck(path):
 class Vlp4d:
     """One simulation on one GPU (context of the C-ABI)."""
 
-    def __init__(self, cfg_or_path, precision=FP64, device=0):
+    def __init__(self, cfg_or_path, precision=FP64, device=0, scheme=SCHEME_SPLINE):
+        """scheme: SCHEME_SPLINE (the spline step of miniapps/vlp4d_mpi) or
+        SCHEME_LAGRANGE (the split-step solver of miniapps/vlp4d, FP64)."""
         self.lib = load()
         h = ctypes.c_void_p()
-        if isinstance(cfg_or_path, (str, bytes, os.PathLike)):
+        if scheme == SCHEME_LAGRANGE:
+            if precision != FP64:
+                raise Vlp4dError(E_ARG, "the Lagrange split-step solver runs in FP64")
+            if isinstance(cfg_or_path, (str, bytes, os.PathLike)):
+                cfg_or_path = parse_deck(cfg_or_path)
+            rc = self.lib.vlp_create_lagrange(ctypes.byref(cfg_or_path), device, ctypes.byref(h))
+        elif isinstance(cfg_or_path, (str, bytes, os.PathLike)):
             rc = self.lib.vlp_create(os.fsencode(cfg_or_path), precision, device, ctypes.byref(h))
         else:
             rc = self.lib.vlp_create_config(ctypes.byref(cfg_or_path), precision, device, ctypes.byref(h))

@@ -1,0 +1,181 @@
+// lagrange.hip -- the Lagrange split-step vlp4d (miniapps/vlp4d/openmp, "VO/").
+//
+// One step is the Strang split of VO/timestep.hpp:11-55: 1-D advections along
+// x and y by dt/2 (displacement dt*v of the cell's own velocity), the density
+// of the result and the FFT Poisson solve (field.hip, shared with the spline
+// path), 1-D advections along vx and vy by dt (displacement dt*E(x, y)), and x,
+// y by dt/2 again.  Every advection is the 6-point order-5 interpolation of
+// VO/advect.hpp:72-86 on a periodic axis (velocity too).  Each pass streams f
+// once (8 B read + 8 B write per cell in FP64): lanes run along x, so the six
+// stencil reads of every lane are coalesced rows that stay in L2 between
+// neighbouring lanes, lines or planes.
+#include "device_common.hpp"
+#include "vlp4d_internal.hpp"
+
+namespace vlp {
+
+namespace {
+
+// The order-5 basis of VO/advect.hpp:72-86, px in [2, 3).
+__device__ __forceinline__ void lag5_basis(double px, double c[6]) {
+  const double pxm2 = px - 2.;
+  const double sq = pxm2 * pxm2;
+  const double p01 = pxm2 * (pxm2 - 1.);
+  c[0] = (-1. / 24) * p01 * (pxm2 + 1.) * (pxm2 - 2.) * (pxm2 - 1.);
+  c[1] = (1. / 24.) * p01 * (pxm2 - 2.) * (5 * sq + pxm2 - 8.);
+  c[2] = (-1. / 12.) * (pxm2 - 1.) * (pxm2 - 2.) * (pxm2 + 1.) * (5 * sq - 3 * pxm2 - 6.);
+  c[3] = (1. / 12.) * pxm2 * (pxm2 + 1.) * (pxm2 - 2.) * (5 * sq - 7 * pxm2 - 4.);
+  c[4] = (-1. / 24.) * p01 * (pxm2 + 1.) * (5 * sq - 11 * pxm2 - 2.);
+  c[5] = (1. / 24.) * p01 * pxm2 * (pxm2 + 1.) * (pxm2 - 2.);
+}
+
+// Foot of a 1-D advection by `dep` on the periodic axis d (VO/advect.hpp:
+// 137-154): the six stencil indices, already wrapped into [0, N), and weights.
+__device__ __forceinline__ void lag_stencil(const Geom &g, int d, double h, double dep, int j[6], double c[6]) {
+  const double hmin = g.minPhy[d], hwidth = g.maxPhy[d] - g.minPhy[d], dh = g.dx[d];
+  const double idh = 1. / dh;
+  const int N = g.n[d];
+  const double hstar = hmin + fmod(hwidth + h - dep - hmin, hwidth);
+  const int ip = (int)floor((hstar - hmin) * idh);
+  lag5_basis(2. + idh * (hstar - (hmin + ip * dh)), c);
+  // the reference wraps with (N + ip - 2 + k) % N; a full modulo keeps any
+  // foot (even beyond one box width) in range
+  int b = (ip - 2) % N;
+  if (b < 0) b += N;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int q = b + k;
+    j[k] = q >= N ? q - N : q;
+  }
+}
+
+}  // namespace
+
+// advect_1D_x (VO/advect.hpp:112-158): lane per cell, x fastest.
+__global__ __launch_bounds__(256) void k_lag_x(const double *__restrict__ in, double *__restrict__ out, Geom g,
+                                               double dt) {
+  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2];
+  const long ncell = (long)N0 * N1 * N2 * g.n[3];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncell) return;
+  const int ix = (int)(t % N0);
+  const long row = t - ix;
+  const int ivx = (int)((t / ((long)N0 * N1)) % N2);
+  const double vx = g.minPhy[2] + ivx * g.dx[2];
+  int j[6];
+  double c[6];
+  lag_stencil(g, 0, g.minPhy[0] + ix * g.dx[0], dt * vx, j, c);
+  double s = 0.;
+#pragma unroll
+  for (int k = 0; k < 6; k++) s += c[k] * in[row + j[k]];
+  out[t] = s;
+}
+
+// advect_1D_y (VO/advect.hpp:160-204): lane per (x, y, vy), looping over vx
+// (the weights depend on y and vy only).  With rho_part, also the velocity
+// sum of the result over vx: rho_part[vy][y][x] (field_rho, VO/field.cpp:5-29).
+__global__ __launch_bounds__(256) void k_lag_y(const double *__restrict__ in, double *__restrict__ out, Geom g,
+                                               double dt, double *__restrict__ rho_part) {
+  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)N0 * N1 * N3) return;
+  const int ix = (int)(t % N0);
+  const int iy = (int)((t / N0) % N1);
+  const int ivy = (int)(t / ((long)N0 * N1));
+  const double vy = g.minPhy[3] + ivy * g.dx[3];
+  int j[6];
+  double c[6];
+  lag_stencil(g, 1, g.minPhy[1] + iy * g.dx[1], dt * vy, j, c);
+  const long plane = (long)N0 * N1;
+  double sum = 0.;
+  for (int ivx = 0; ivx < N2; ivx++) {
+    const long base = ((long)ivx + (long)N2 * ivy) * plane + ix;
+    double s = 0.;
+#pragma unroll
+    for (int k = 0; k < 6; k++) s += c[k] * in[base + (long)j[k] * N0];
+    out[base + (long)iy * N0] = s;
+    sum += s;
+  }
+  if (rho_part) rho_part[t] = sum;
+}
+
+// advect_1D_vx / _vy (VO/advect.hpp:206-301): lane per cell, displacement
+// dt*E(x, y) along velocity axis D (2 or 3).
+template <int D>
+__global__ __launch_bounds__(256) void k_lag_v(const double *__restrict__ in, double *__restrict__ out, Geom g,
+                                               const double *__restrict__ e, double dt) {
+  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2];
+  const long plane = (long)N0 * N1;
+  const long ncell = plane * N2 * g.n[3];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncell) return;
+  const long pxy = t % plane;
+  const int iv = D == 2 ? (int)((t / plane) % N2) : (int)(t / (plane * N2));
+  const long stride = D == 2 ? plane : plane * N2;
+  const long base = t - (long)iv * stride;
+  int j[6];
+  double c[6];
+  lag_stencil(g, D, g.minPhy[D] + iv * g.dx[D], dt * e[pxy], j, c);
+  double s = 0.;
+#pragma unroll
+  for (int k = 0; k < 6; k++) s += c[k] * in[base + (long)j[k] * stride];
+  out[t] = s;
+}
+
+// Initial conditions of VO/init.cpp:144-285 (TSI20 differs from the spline
+// miniapp's).
+__global__ __launch_bounds__(256) void k_lag_init(double *__restrict__ f, Geom g) {
+  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2];
+  const long ncell = (long)N0 * N1 * N2 * g.n[3];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncell) return;
+  const int ix = (int)(t % N0), iy = (int)((t / N0) % N1), ivx = (int)((t / ((long)N0 * N1)) % N2),
+            ivy = (int)(t / ((long)N0 * N1 * N2));
+  const double x = g.minPhy[0] + ix * g.dx[0], y = g.minPhy[1] + iy * g.dx[1];
+  const double vx = g.minPhy[2] + ivx * g.dx[2], vy = g.minPhy[3] + ivy * g.dx[3];
+  double v = 0.;
+  if (g.idcase == 2 || g.idcase == 5) {  // CYL02 (x, vx) / CYL05 (y, vy)
+    const double PERIOD = 0.5 * M_PI, AMPLI = 4, cc = 0.50 * (6. / 16.), rc = 0.50 * (4. / 16.);
+    const double xx = g.idcase == 2 ? x : y, vv = g.idcase == 2 ? vx : vy;
+    double hv = 0.0, hx = 0.0;
+    if ((vv <= cc + rc) && (vv >= cc - rc)) hv = cos(PERIOD * ((vv - cc) / rc));
+    else if ((vv <= -cc + rc) && (vv >= -cc - rc)) hv = -cos(PERIOD * ((vv + cc) / rc));
+    if ((xx <= cc + rc) && (xx >= cc - rc)) hx = cos(PERIOD * ((xx - cc) / rc));
+    else if ((xx <= -cc + rc) && (xx >= -cc - rc)) hx = -cos(PERIOD * ((xx + cc) / rc));
+    v = AMPLI * hx * hv;
+  } else if (g.idcase == 10) {  // SLD10
+    const double sum = vx * vx + vy * vy;
+    v = (1. / (2 * M_PI)) * exp(-0.5 * (sum)) * (1 + 0.05 * (cos(0.5 * x) * cos(0.5 * y)));
+  } else {  // TSI20
+    const double xi = 0.90, alpha = 0.05;
+    const double sum = vx * vx + vy * vy;
+    v = (1 + alpha * (cos(0.5 * x))) * 1 / (2 * M_PI) * ((2 - 2 * xi) / (3 - 2 * xi)) * (1 + .5 * vx * vx / (1 - xi)) *
+        exp(-.5 * sum);
+  }
+  f[t] = v;
+}
+
+static unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
+
+void launch_lag_init(const Geom &g, double *f, hipStream_t s) {
+  const long n = (long)g.n[0] * g.n[1] * g.n[2] * g.n[3];
+  hipLaunchKernelGGL(k_lag_init, dim3(blocks(n)), dim3(256), 0, s, f, g);
+}
+
+void launch_lag_x(const Geom &g, const double *in, double *out, double dt, hipStream_t s) {
+  const long n = (long)g.n[0] * g.n[1] * g.n[2] * g.n[3];
+  hipLaunchKernelGGL(k_lag_x, dim3(blocks(n)), dim3(256), 0, s, in, out, g, dt);
+}
+
+void launch_lag_y(const Geom &g, const double *in, double *out, double dt, double *rho_part, hipStream_t s) {
+  const long n = (long)g.n[0] * g.n[1] * g.n[3];
+  hipLaunchKernelGGL(k_lag_y, dim3(blocks(n)), dim3(256), 0, s, in, out, g, dt, rho_part);
+}
+
+void launch_lag_v(const Geom &g, int d, const double *in, double *out, const double *e, double dt, hipStream_t s) {
+  const long n = (long)g.n[0] * g.n[1] * g.n[2] * g.n[3];
+  if (d == 2) hipLaunchKernelGGL(k_lag_v<2>, dim3(blocks(n)), dim3(256), 0, s, in, out, g, e, dt);
+  else hipLaunchKernelGGL(k_lag_v<3>, dim3(blocks(n)), dim3(256), 0, s, in, out, g, e, dt);
+}
+
+}  // namespace vlp

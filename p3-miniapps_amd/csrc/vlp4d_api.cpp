@@ -33,6 +33,7 @@ struct vlp_ctx {
   vlp_config cfg{};
   Geom g{};
   int precision = VLP_FP64;
+  int scheme = VLP_SCHEME_SPLINE;  // VLP_SCHEME_LAGRANGE: the split-step Lagrange solver (lagrange.hip)
   int device = 0;
   hipStream_t stream = nullptr;
   size_t esz = 8;
@@ -299,8 +300,49 @@ int enqueue_step(vlp_ctx *c, int *iter_dev) {
   return VLP_OK;
 }
 
+// onetimestep of the Lagrange split-step solver (VO/timestep.hpp:11-55):
+// x/2, y/2 (+ density), field solve with the step's diagnostics, vx, vy, x/2,
+// y/2; the result ends in f.  Timers: the x/y passes count as advec2D, the
+// velocity passes as advec4D.
+int lag_step_once(vlp_ctx *c) {
+  hipEvent_t *ev = c->ev;
+  const Geom &g = c->g;
+  const double dt = c->cfg.dt;
+  double *f = (double *)c->f, *t = (double *)c->fnext;
+  hipStream_t s = c->stream;
+  HIP_TRY(c, hipEventRecord(ev[0], s));
+  vlp::launch_lag_x(g, f, t, 0.5 * dt, s);
+  vlp::launch_lag_y(g, t, f, 0.5 * dt, c->rho_part, s);
+  c->rho_outer = g.n[3];
+  HIP_TRY(c, hipEventRecord(ev[1], s));
+  int rc = field(c, c->iter + 1, true, false);
+  if (rc) return rc;
+  HIP_TRY(c, hipEventRecord(ev[2], s));
+  vlp::launch_lag_v(g, 2, f, t, c->fb.ex, dt, s);
+  vlp::launch_lag_v(g, 3, t, f, c->fb.ey, dt, s);
+  HIP_TRY(c, hipEventRecord(ev[3], s));
+  vlp::launch_lag_x(g, f, t, 0.5 * dt, s);
+  vlp::launch_lag_y(g, t, f, 0.5 * dt, nullptr, s);
+  HIP_TRY(c, hipEventRecord(ev[4], s));
+  rc = check_launch(c, "step");
+  if (rc) return rc;
+  c->iter++;
+  rc = sync_and_check(c);
+  if (rc) return rc;
+  c->tsec[VLP_T_ADVEC2D] += 1e-3 * (ev_ms(ev[0], ev[1]) + ev_ms(ev[3], ev[4]));
+  c->tsec[VLP_T_FIELD] += 1e-3 * ev_ms(ev[1], ev[2]);
+  c->tsec[VLP_T_ADVEC4D] += 1e-3 * ev_ms(ev[2], ev[3]);
+  c->tcalls[VLP_T_ADVEC2D] += 2;
+  c->tcalls[VLP_T_ADVEC4D]++;
+  c->tcalls[VLP_T_FIELD]++;
+  c->tcalls[VLP_T_FOURIER]++;
+  c->tcalls[VLP_T_DIAG]++;
+  return VLP_OK;
+}
+
 // One onetimestep + swap, events between phases.
 int step_once(vlp_ctx *c) {
+  if (c->scheme == VLP_SCHEME_LAGRANGE) return lag_step_once(c);
   hipEvent_t *ev = c->ev;
   int rc = enqueue_step(c, nullptr);
   if (rc) return rc;
@@ -330,6 +372,14 @@ int usable(vlp_ctx *c) {
   if (!c) return set_err(nullptr, VLP_E_ARG, "null context");
   if (c->broken) return set_err(c, VLP_E_ARG, "context unusable after an earlier error: " + c->err);
   if (hipSetDevice(c->device) != hipSuccess) return set_err(c, VLP_E_HIP, "hipSetDevice failed");
+  return VLP_OK;
+}
+
+// Calls that exist for the spline solver only (phases, tile tuner, adjoint).
+int spline_only(vlp_ctx *c) {
+  int rc = usable(c);
+  if (rc) return rc;
+  if (c->scheme != VLP_SCHEME_SPLINE) return set_err(c, VLP_E_ARG, "not available for a Lagrange split-step context");
   return VLP_OK;
 }
 
@@ -363,7 +413,8 @@ int download(vlp_ctx *c, const void *dev, size_t n, double *host) {
 // (VMO/Vlp4d.cpp:30-34 for iter 0).
 int refresh_from_f(vlp_ctx *c) {
   if (c->precision == VLP_FP64) run_moments<double>(c); else run_moments<float>(c);
-  int rc = field(c, c->iter, true, true);
+  // the Lagrange solver records no L2 norm (VO/diags.cpp:13-39)
+  int rc = field(c, c->iter, true, c->scheme != VLP_SCHEME_LAGRANGE);
   if (rc) return rc;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return VLP_OK;
@@ -411,11 +462,22 @@ int vlp_create(const char *dat_path, int precision, int device, vlp_ctx **out) {
   return vlp_create_config(&cfg, precision, device, out);
 }
 
+static int create_ctx(const vlp_config *cfg_in, int precision, int device, int scheme, vlp_ctx **out);
+
 int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_ctx **out) {
+  return create_ctx(cfg_in, precision, device, VLP_SCHEME_SPLINE, out);
+}
+
+int vlp_create_lagrange(const vlp_config *cfg_in, int device, vlp_ctx **out) {
+  return create_ctx(cfg_in, VLP_FP64, device, VLP_SCHEME_LAGRANGE, out);
+}
+
+static int create_ctx(const vlp_config *cfg_in, int precision, int device, int scheme, vlp_ctx **out) {
   if (!cfg_in || !out) return set_err(nullptr, VLP_E_ARG, "null argument");
   *out = nullptr;
   if (precision != VLP_FP64 && precision != VLP_FP32 && precision != VLP_FP32_ACC)
     return set_err(nullptr, VLP_E_ARG, "precision must be VLP_FP64 (64), VLP_FP32 (32) or VLP_FP32_ACC (33)");
+  const bool lag = scheme == VLP_SCHEME_LAGRANGE;
   vlp_config cfg = *cfg_in;
   derive_dx(&cfg);
   int rc = check_config(&cfg);
@@ -427,6 +489,7 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
   vlp_ctx *c = new vlp_ctx();
   c->cfg = cfg;
   c->precision = precision;
+  c->scheme = scheme;
   c->device = device;
   c->esz = precision == VLP_FP64 ? sizeof(double) : sizeof(float);
   for (int i = 0; i < 4; i++) {
@@ -439,12 +502,13 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
   c->g.idcase = cfg.idcase;
   const size_t N0 = cfg.nxmax[0], N1 = cfg.nxmax[1], N2 = cfg.nxmax[2], N3 = cfg.nxmax[3];
   c->n_f = N0 * N1 * N2 * N3;
-  c->n_cxy = (N0 + 4) * (N1 + 4) * N2 * N3;
-  c->n_tv = (N0 + 4) * (N1 + 4) * (N2 + 4) * N3;
-  c->n_c4 = (N0 + 4) * (N1 + 4) * (N2 + 4) * (N3 + 4);
-  c->n_c4s = (N0 + 6) * (N1 + 6) * (N2 + 6) * (N3 + 6) + 64;
+  // the Lagrange solver keeps no spline coefficients
+  c->n_cxy = lag ? 0 : (N0 + 4) * (N1 + 4) * N2 * N3;
+  c->n_tv = lag ? 0 : (N0 + 4) * (N1 + 4) * (N2 + 4) * N3;
+  c->n_c4 = lag ? 0 : (N0 + 4) * (N1 + 4) * (N2 + 4) * (N3 + 4);
+  c->n_c4s = lag ? 0 : (N0 + 6) * (N1 + 6) * (N2 + 6) * (N3 + 6) + 64;
   c->n_l2 = vlp::l2_partials(c->g);
-  c->xy_groups = vlp::xy_fused_groups(c->g, c->esz);
+  c->xy_groups = lag ? 0 : vlp::xy_fused_groups(c->g, c->esz);
   c->rho_outer = N3;
   const size_t nxh = N0 / 2 + 1;
   auto fail = [&](int code) {
@@ -464,9 +528,11 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
     if (hipEventCreate(&e) != hipSuccess) return fail(set_err(nullptr, VLP_E_HIP, "hipEventCreate failed"));
   ALLOC(c->f, c->n_f * c->esz);
   ALLOC(c->fnext, c->n_f * c->esz);
-  ALLOC(c->cxy, c->n_cxy * c->esz);
-  ALLOC(c->tv, c->n_tv * c->esz);
-  ALLOC(c->c4, c->n_c4s * c->esz);
+  if (!lag) {
+    ALLOC(c->cxy, c->n_cxy * c->esz);
+    ALLOC(c->tv, c->n_tv * c->esz);
+    ALLOC(c->c4, c->n_c4s * c->esz);
+  }
   ALLOC(c->rho_part, N0 * N1 * N3 * (c->xy_groups > 1 ? (size_t)c->xy_groups : 1) * sizeof(double));
   ALLOC(c->l2_part, (size_t)c->n_l2 * sizeof(double));
   ALLOC(c->fb.rho, N0 * N1 * sizeof(double));
@@ -490,7 +556,7 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
       hipMemcpy(c->fb.twy, twy.data(), twy.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(c->fb.diag, 0, 5 * (size_t)(cfg.nbiter + 1) * sizeof(double)) != hipSuccess ||
       hipMemset(c->d_err, 0, sizeof(int)) != hipSuccess || hipMemset(c->f, 0, c->n_f * c->esz) != hipSuccess ||
-      hipMemset(c->c4, 0, c->n_c4s * c->esz) != hipSuccess ||
+      (!lag && hipMemset(c->c4, 0, c->n_c4s * c->esz) != hipSuccess) ||
       hipMemset(c->fb.rho, 0, N0 * N1 * sizeof(double)) != hipSuccess ||
       hipMemset(c->fb.ex, 0, N0 * N1 * sizeof(double)) != hipSuccess ||
       hipMemset(c->fb.ey, 0, N0 * N1 * sizeof(double)) != hipSuccess ||
@@ -498,7 +564,8 @@ int vlp_create_config(const vlp_config *cfg_in, int precision, int device, vlp_c
     return fail(set_err(nullptr, VLP_E_HIP, "device initialisation failed"));
   // init(): VMO/Init.cpp:370-372 then VMO/Vlp4d.cpp:30-34
   const auto t0 = std::chrono::steady_clock::now();
-  if (precision == VLP_FP64) vlp::launch_init_case<double>(c->g, (double *)c->f, c->stream);
+  if (lag) vlp::launch_lag_init(c->g, (double *)c->f, c->stream);  // VO/init.cpp:144-285
+  else if (precision == VLP_FP64) vlp::launch_init_case<double>(c->g, (double *)c->f, c->stream);
   else vlp::launch_init_case<float>(c->g, (float *)c->f, c->stream);
   rc = refresh_from_f(c);
   if (rc) return fail(rc);
@@ -567,6 +634,14 @@ int vlp_save(vlp_ctx *c, const char *dir) {
   HIP_TRY(c, hipMemcpyAsync(h.data(), c->fb.diag, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   const std::string d = dir ? dir : ".";
+  if (c->scheme == VLP_SCHEME_LAGRANGE) {  // VO/diags.cpp:41-52: t, nrj, mass
+    FILE *f = fopen((d + "/nrj.out").c_str(), "w");
+    if (!f) return set_err(c, VLP_E_FILE, "cannot write nrj.out");
+    for (int it = 0; it <= c->cfg.nbiter; ++it)
+      fprintf(f, "%17.13e %17.13e %17.13e\n", it * c->cfg.dt, h[it], h[3 * nb + it]);
+    fclose(f);
+    return VLP_OK;
+  }
   FILE *f = fopen((d + "/nrj.out").c_str(), "w");  // VMO/Diags.cpp:403-411
   if (!f) return set_err(c, VLP_E_FILE, "cannot write nrj.out");
   for (int it = 0; it <= c->cfg.nbiter; ++it)
@@ -640,7 +715,7 @@ int vlp_get_buffer(vlp_ctx *c, int which, double *host, size_t n) {
 }
 
 int vlp_spline_xy(vlp_ctx *c) {
-  int rc = usable(c);
+  int rc = spline_only(c);
   if (rc) return rc;
   if (c->precision == VLP_FP64) run_spline_xy<double>(c); else run_spline_xy<float>(c);
   rc = check_launch(c, "spline_xy");
@@ -650,7 +725,7 @@ int vlp_spline_xy(vlp_ctx *c) {
 }
 
 int vlp_advect2d_poisson(vlp_ctx *c) {
-  int rc = usable(c);
+  int rc = spline_only(c);
   if (rc) return rc;
   if (c->iter >= c->cfg.nbiter) return set_err(c, VLP_E_ARG, "iteration count would exceed nbiter");
   if (c->precision == VLP_FP64) run_advect2d<double>(c); else run_advect2d<float>(c);
@@ -660,7 +735,7 @@ int vlp_advect2d_poisson(vlp_ctx *c) {
 }
 
 int vlp_spline_vxvy(vlp_ctx *c) {
-  int rc = usable(c);
+  int rc = spline_only(c);
   if (rc) return rc;
   if (c->precision == VLP_FP64) run_spline_vxvy<double>(c); else run_spline_vxvy<float>(c);
   rc = check_launch(c, "spline_vxvy");
@@ -670,7 +745,7 @@ int vlp_spline_vxvy(vlp_ctx *c) {
 }
 
 int vlp_advect4d_poisson(vlp_ctx *c) {
-  int rc = usable(c);
+  int rc = spline_only(c);
   if (rc) return rc;
   if (c->iter >= c->cfg.nbiter) return set_err(c, VLP_E_ARG, "iteration count would exceed nbiter");
   const bool f64 = c->precision == VLP_FP64;
@@ -698,6 +773,7 @@ const char *vlp_last_error(const vlp_ctx *c) { return c ? c->err.c_str() : g_las
 
 int vlp_kernel_variants(const vlp_ctx *c) {
   if (!c) return -1;
+  if (c->scheme == VLP_SCHEME_LAGRANGE) return 0;
   int bits = vlp::kernel_variants(c->g, c->precision != VLP_FP64, c->precision == VLP_FP32_ACC);
   if ((bits & 128) && c->ys_tvx) bits = (bits & ~(15 << 8)) | (c->ys_tvx << 8);
   return bits;
@@ -712,7 +788,7 @@ int vlp_kernel_variants(const vlp_ctx *c) {
 // (:127-148).  The launches write fnext, which
 // the next step overwrites, and report errors to a scratch flag.
 int vlp_autotune(vlp_ctx *c, const char *csv_path) {
-  int rc = usable(c);
+  int rc = spline_only(c);
   if (rc) return rc;
   HIP_TRY(c, hipSetDevice(c->device));
   if (getenv("VLP4D_YS_TVX") || getenv("VLP4D_YS_NBUF") || getenv("VLP4D_YSTREAM")) return VLP_OK;
@@ -775,7 +851,7 @@ int vlp_autotune(vlp_ctx *c, const char *csv_path) {
 // (adjoint.hip).  The coefficient, density and field buffers are left
 // holding intermediate values; the next step recomputes them from f.
 int vlp_adjoint_step(vlp_ctx *c, const double *g_host, double *grad_host, size_t n) {
-  int rc = usable(c);
+  int rc = spline_only(c);
   if (rc) return rc;
   if (c->precision != VLP_FP64) return set_err(c, VLP_E_ARG, "the adjoint is implemented for VLP_FP64 contexts");
   if (!g_host || !grad_host || n != c->n_f) return set_err(c, VLP_E_SIZE, "f buffer size mismatch");
@@ -961,6 +1037,8 @@ int vlp_ensemble_create(vlp_ctx *const *members, int n, vlp_ensemble **out) {
     if (!members[i]) return set_err(nullptr, VLP_E_ARG, "null member");
     if (members[i]->broken) return set_err(nullptr, VLP_E_ARG, "a member is in an error state");
     if (members[i]->device != members[0]->device) return set_err(nullptr, VLP_E_ARG, "members on different devices");
+    if (members[i]->scheme != VLP_SCHEME_SPLINE)
+      return set_err(nullptr, VLP_E_ARG, "ensembles take spline-solver contexts only");
     for (int j = 0; j < i; j++)
       if (members[j] == members[i]) return set_err(nullptr, VLP_E_ARG, "a context appears twice");
   }

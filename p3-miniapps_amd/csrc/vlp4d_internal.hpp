@@ -91,6 +91,12 @@ long l2_partials(const Geom &g);
 int kernel_variants(const Geom &g, bool fp32_storage, bool arith32);
 
 // Adjoint of one step (adjoint.hip), FP64.
+// Lagrange split-step vlp4d (lagrange.hip), FP64, arrays without halo.
+void launch_lag_init(const Geom &g, double *f, hipStream_t s);
+void launch_lag_x(const Geom &g, const double *in, double *out, double dt, hipStream_t s);
+void launch_lag_y(const Geom &g, const double *in, double *out, double dt, double *rho_part, hipStream_t s);
+void launch_lag_v(const Geom &g, int d, const double *in, double *out, const double *e, double dt, hipStream_t s);
+
 void launch_adj_advect4d(const Geom &g, const double *c4, const double *ex, const double *ey, double dt,
                          const double *gin, double *lc4, double *lE, bool field_term, hipStream_t s);
 void launch_axis_contract(const double *Mt, int Min, int Nout, const double *in, double *out, long O, long I,
