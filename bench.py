@@ -11,6 +11,7 @@ steps (inputs resident in HBM).  Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision 64|32|33]
                   [--n 128] [--no-cpu-baseline]
+  python bench.py --lagrange [--n 128] # the Lagrange split-step solver (miniapps/vlp4d)
   python bench.py --calibrate-copy     # plain copy kernel of known size, for
                                        # scaling rocprofv3 FETCH_SIZE/WRITE_SIZE
 """
@@ -201,6 +202,66 @@ def ensemble_bench(pkg, steps, seed=20261202):
     }
 
 
+def lagrange_cpu_baseline(n, steps=1):
+    """The Lagrange oracle (C/OpenMP port of miniapps/vlp4d/openmp) on the
+    host cores: `steps` onetimestep(s) of the same deck, timed in-process."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    cores = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    oracle.load()
+    cfg = oracle.make_config([n] * 4, SLD10["minphy"], SLD10["maxphy"], SLD10["idcase"], SLD10["dt"], steps)
+    st = oracle.LagState(cfg)
+    t0 = time.perf_counter()
+    st.step(steps)
+    sec = (time.perf_counter() - t0) / steps
+    st.close()
+    return {"value": n ** 4 / sec, "unit": "cell-updates/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
+            "kind": "port", "sample": f"{steps} onetimestep(s) of the SLD10 deck at {n}^4, oracle/vlp4d_lag.c "
+                                      f"(FP64, OpenMP, {sec:.2f} s/step)"}
+
+
+def lagrange_bench(pkg, n, steps, warmup, with_cpu):
+    """The Lagrange split-step solver (miniapps/vlp4d) on the SLD10 deck at
+    n^4: six 1-D order-5 advections per step (x/2, y/2, vx, vy, x/2, y/2),
+    each streaming f once (16 B per cell in FP64), plus the field solve."""
+    nb = warmup + steps
+    cfg = pkg.VlpConfig.make(n, SLD10["minphy"], SLD10["maxphy"], SLD10["idcase"], SLD10["dt"], nb)
+    sim = pkg.Vlp4d(cfg, scheme=pkg.SCHEME_LAGRANGE)
+    if warmup:
+        sim.step(warmup)
+    tb = sim.timers()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    dt = time.perf_counter() - t0
+    ta = sim.timers()
+    d = sim.diags()
+    sim.close()
+    ph = {lab: (ta[lab][0] - tb[lab][0]) / steps for lab in ("advec2D", "field", "advec4D")}
+    # per launch: the x/y phase holds 4 launches, the velocity phase 2
+    per = {"k_lag_x/k_lag_y": ph["advec2D"] / 4, "k_lag_v": ph["advec4D"] / 2}
+    kern = max(per, key=per.get)
+    launch_bytes = 16 * n ** 4
+    achieved = launch_bytes / per[kern] / 1e9
+    res = {
+        "metric": f"4D cell updates/s, Lagrange split-step vlp4d at {n}^4 + % of HBM roofline",
+        "value": n ** 4 * steps / dt, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * dt / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic: SLD10 Landau-damping initial condition generated on the GPU",
+        "config": {"workload": f"vlp4d Lagrange split step (miniapps/vlp4d), SLD10 deck at {n}^4",
+                   "grid": [n] * 4, "idcase": 10, "dt": SLD10["dt"]},
+        "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": launch_bytes, "avg_launch_ms": 1e3 * per[kern]},
+        "step_hbm_fraction": 6 * launch_bytes / (dt / steps) / 1e9 / HBM_PEAK_GBS,
+        "phases_ms": {k: round(1e3 * v, 4) for k, v in ph.items()},
+        "final_nrj": float(d["nrj"][-1]),
+    }
+    if with_cpu:
+        res["cpu_baseline"] = lagrange_cpu_baseline(n)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +275,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--ensemble", action="store_true",
                     help="batched-ensemble workload (48 uneven Landau solves); --steps is the solve length")
+    ap.add_argument("--lagrange", action="store_true",
+                    help="the Lagrange split-step solver of miniapps/vlp4d on the same deck")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -225,6 +288,9 @@ def main():
     pkg = load_pkg()
     if a.ensemble:
         print(json.dumps(ensemble_bench(pkg, a.steps)))
+        return
+    if a.lagrange:
+        print(json.dumps(lagrange_bench(pkg, a.n, a.steps, a.warmup, not a.no_cpu_baseline)))
         return
     n = a.n
     nb = a.warmup + a.steps

@@ -8,7 +8,9 @@
 // VO/advect.hpp:72-86 on a periodic axis (velocity too).  Each pass streams f
 // once (8 B read + 8 B write per cell in FP64): lanes run along x, so the six
 // stencil reads of every lane are coalesced rows that stay in L2 between
-// neighbouring lanes, lines or planes.
+// neighbouring lanes, lines or planes.  Each lane computes its stencil (the
+// fmod/floor foot and the six weights) once and applies it along the axis the
+// weights do not depend on.
 #include "device_common.hpp"
 #include "vlp4d_internal.hpp"
 
@@ -51,24 +53,39 @@ __device__ __forceinline__ void lag_stencil(const Geom &g, int d, double h, doub
 
 }  // namespace
 
-// advect_1D_x (VO/advect.hpp:112-158): lane per cell, x fastest.
+// Logical block of hardware block b: block b runs on XCD b % 8, so with the
+// grid a multiple of 8 each XCD takes one contiguous range of logical blocks
+// and the stencil neighbours of a block (the next rows, planes or lines) sit
+// in the same L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned grid) {
+  return (grid & 7u) ? b : (b & 7u) * (grid >> 3) + (b >> 3);
+}
+
+// advect_1D_x (VO/advect.hpp:112-158).  Lane per (x, vx, vy), x fastest; the
+// weights depend on x and vx only, so each lane computes them once and runs
+// down the y rows of its (vx, vy) plane.
 __global__ __launch_bounds__(256) void k_lag_x(const double *__restrict__ in, double *__restrict__ out, Geom g,
                                                double dt) {
-  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2];
-  const long ncell = (long)N0 * N1 * N2 * g.n[3];
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncell) return;
-  const int ix = (int)(t % N0);
-  const long row = t - ix;
-  const int ivx = (int)((t / ((long)N0 * N1)) % N2);
+  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= (unsigned)N0 * N2 * N3) return;
+  const int ix = (int)(t % (unsigned)N0);
+  const unsigned r = t / (unsigned)N0;  // ivx + N2 * ivy
+  const int ivx = (int)(r % (unsigned)N2);
   const double vx = g.minPhy[2] + ivx * g.dx[2];
   int j[6];
   double c[6];
   lag_stencil(g, 0, g.minPhy[0] + ix * g.dx[0], dt * vx, j, c);
-  double s = 0.;
+  const double *src = in + (size_t)r * N1 * N0;
+  double *dst = out + (size_t)r * N1 * N0 + ix;
+#pragma unroll 2
+  for (int iy = 0; iy < N1; iy++) {
+    const double *row = src + (size_t)iy * N0;
+    double s = 0.;
 #pragma unroll
-  for (int k = 0; k < 6; k++) s += c[k] * in[row + j[k]];
-  out[t] = s;
+    for (int k = 0; k < 6; k++) s += c[k] * row[j[k]];
+    dst[(size_t)iy * N0] = s;
+  }
 }
 
 // advect_1D_y (VO/advect.hpp:160-204): lane per (x, y, vy), looping over vx
@@ -77,49 +94,60 @@ __global__ __launch_bounds__(256) void k_lag_x(const double *__restrict__ in, do
 __global__ __launch_bounds__(256) void k_lag_y(const double *__restrict__ in, double *__restrict__ out, Geom g,
                                                double dt, double *__restrict__ rho_part) {
   const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)N0 * N1 * N3) return;
-  const int ix = (int)(t % N0);
-  const int iy = (int)((t / N0) % N1);
-  const int ivy = (int)(t / ((long)N0 * N1));
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const unsigned plane = (unsigned)N0 * N1;
+  if (t >= plane * N3) return;
+  const unsigned pxy = t % plane;
+  const int ix = (int)(pxy % (unsigned)N0);
+  const int iy = (int)(pxy / (unsigned)N0);
+  const int ivy = (int)(t / plane);
   const double vy = g.minPhy[3] + ivy * g.dx[3];
   int j[6];
   double c[6];
   lag_stencil(g, 1, g.minPhy[1] + iy * g.dx[1], dt * vy, j, c);
-  const long plane = (long)N0 * N1;
+  const double *src = in + (size_t)ivy * N2 * plane + ix;
+  double *dst = out + (size_t)ivy * N2 * plane + pxy;
   double sum = 0.;
+#pragma unroll 2
   for (int ivx = 0; ivx < N2; ivx++) {
-    const long base = ((long)ivx + (long)N2 * ivy) * plane + ix;
+    const double *p = src + (size_t)ivx * plane;
     double s = 0.;
 #pragma unroll
-    for (int k = 0; k < 6; k++) s += c[k] * in[base + (long)j[k] * N0];
-    out[base + (long)iy * N0] = s;
+    for (int k = 0; k < 6; k++) s += c[k] * p[j[k] * N0];
+    dst[(size_t)ivx * plane] = s;
     sum += s;
   }
   if (rho_part) rho_part[t] = sum;
 }
 
-// advect_1D_vx / _vy (VO/advect.hpp:206-301): lane per cell, displacement
-// dt*E(x, y) along velocity axis D (2 or 3).
+// advect_1D_vx / _vy (VO/advect.hpp:206-301): displacement dt*E(x, y) along
+// velocity axis D (2 or 3).  Lane per (x, y, v_D), x fastest; the weights do
+// not depend on the other velocity, which the lane loops over.
 template <int D>
 __global__ __launch_bounds__(256) void k_lag_v(const double *__restrict__ in, double *__restrict__ out, Geom g,
                                                const double *__restrict__ e, double dt) {
-  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2];
-  const long plane = (long)N0 * N1;
-  const long ncell = plane * N2 * g.n[3];
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncell) return;
-  const long pxy = t % plane;
-  const int iv = D == 2 ? (int)((t / plane) % N2) : (int)(t / (plane * N2));
-  const long stride = D == 2 ? plane : plane * N2;
-  const long base = t - (long)iv * stride;
+  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const int ND = D == 2 ? N2 : N3, NO = D == 2 ? N3 : N2;  // advected axis, looped axis
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const unsigned plane = (unsigned)N0 * N1;
+  if (t >= plane * ND) return;
+  const unsigned pxy = t % plane;
+  const int iv = (int)(t / plane);
   int j[6];
   double c[6];
   lag_stencil(g, D, g.minPhy[D] + iv * g.dx[D], dt * e[pxy], j, c);
-  double s = 0.;
+  // strides (in cells) of the advected and the looped velocity axes
+  const size_t sD = D == 2 ? plane : (size_t)plane * N2, sO = D == 2 ? (size_t)plane * N2 : plane;
+  const double *src = in + pxy;
+  double *dst = out + pxy + (size_t)iv * sD;
+#pragma unroll 2
+  for (int io = 0; io < NO; io++) {
+    const double *p = src + (size_t)io * sO;
+    double s = 0.;
 #pragma unroll
-  for (int k = 0; k < 6; k++) s += c[k] * in[base + (long)j[k] * stride];
-  out[t] = s;
+    for (int k = 0; k < 6; k++) s += c[k] * p[(size_t)j[k] * sD];
+    dst[(size_t)io * sO] = s;
+  }
 }
 
 // Initial conditions of VO/init.cpp:144-285 (TSI20 differs from the spline
@@ -163,7 +191,7 @@ void launch_lag_init(const Geom &g, double *f, hipStream_t s) {
 }
 
 void launch_lag_x(const Geom &g, const double *in, double *out, double dt, hipStream_t s) {
-  const long n = (long)g.n[0] * g.n[1] * g.n[2] * g.n[3];
+  const long n = (long)g.n[0] * g.n[2] * g.n[3];
   hipLaunchKernelGGL(k_lag_x, dim3(blocks(n)), dim3(256), 0, s, in, out, g, dt);
 }
 
@@ -173,7 +201,7 @@ void launch_lag_y(const Geom &g, const double *in, double *out, double dt, doubl
 }
 
 void launch_lag_v(const Geom &g, int d, const double *in, double *out, const double *e, double dt, hipStream_t s) {
-  const long n = (long)g.n[0] * g.n[1] * g.n[2] * g.n[3];
+  const long n = (long)g.n[0] * g.n[1] * g.n[d];
   if (d == 2) hipLaunchKernelGGL(k_lag_v<2>, dim3(blocks(n)), dim3(256), 0, s, in, out, g, e, dt);
   else hipLaunchKernelGGL(k_lag_v<3>, dim3(blocks(n)), dim3(256), 0, s, in, out, g, e, dt);
 }
