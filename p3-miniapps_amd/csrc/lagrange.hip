@@ -11,6 +11,8 @@
 // neighbouring lanes, lines or planes.  Each lane computes its stencil (the
 // fmod/floor foot and the six weights) once and applies it along the axis the
 // weights do not depend on.
+#include <cstdlib>
+
 #include "device_common.hpp"
 #include "vlp4d_internal.hpp"
 
@@ -121,33 +123,205 @@ __global__ __launch_bounds__(256) void k_lag_y(const double *__restrict__ in, do
 }
 
 // advect_1D_vx / _vy (VO/advect.hpp:206-301): displacement dt*E(x, y) along
-// velocity axis D (2 or 3).  Lane per (x, y, v_D), x fastest; the weights do
-// not depend on the other velocity, which the lane loops over.
+// velocity axis D (2 or 3).  Lane per (x, y, other velocity), x fastest; the
+// lane runs along the advected axis.  The shift dt*E is the same for the
+// whole line, so the foot of point iv is that of point 0 moved by iv cells:
+// one set of weights, and a window of six values that slides by one load per
+// output (the reference recomputes the foot per point; the two agree to
+// rounding, and where rounding moves a foot across a node the interpolant,
+// continuous there, gives the same value).
 template <int D>
 __global__ __launch_bounds__(256) void k_lag_v(const double *__restrict__ in, double *__restrict__ out, Geom g,
                                                const double *__restrict__ e, double dt) {
   const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
-  const int ND = D == 2 ? N2 : N3, NO = D == 2 ? N3 : N2;  // advected axis, looped axis
+  const int ND = D == 2 ? N2 : N3, NO = D == 2 ? N3 : N2;  // advected axis, other axis
   const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const unsigned plane = (unsigned)N0 * N1;
-  if (t >= plane * ND) return;
+  if (t >= plane * NO) return;
   const unsigned pxy = t % plane;
-  const int iv = (int)(t / plane);
+  const int io = (int)(t / plane);
   int j[6];
   double c[6];
-  lag_stencil(g, D, g.minPhy[D] + iv * g.dx[D], dt * e[pxy], j, c);
-  // strides (in cells) of the advected and the looped velocity axes
+  lag_stencil(g, D, g.minPhy[D], dt * e[pxy], j, c);
+  // strides (in cells) of the advected and the other velocity axes
   const size_t sD = D == 2 ? plane : (size_t)plane * N2, sO = D == 2 ? (size_t)plane * N2 : plane;
-  const double *src = in + pxy;
-  double *dst = out + pxy + (size_t)iv * sD;
-#pragma unroll 2
-  for (int io = 0; io < NO; io++) {
-    const double *p = src + (size_t)io * sO;
-    double s = 0.;
-#pragma unroll
-    for (int k = 0; k < 6; k++) s += c[k] * p[(size_t)j[k] * sD];
-    dst[(size_t)io * sO] = s;
+  const double *src = in + pxy + (size_t)io * sO;
+  double *dst = out + pxy + (size_t)io * sO;
+  double w0 = src[(size_t)j[0] * sD], w1 = src[(size_t)j[1] * sD], w2 = src[(size_t)j[2] * sD];
+  double w3 = src[(size_t)j[3] * sD], w4 = src[(size_t)j[4] * sD], w5 = src[(size_t)j[5] * sD];
+  int nxt = j[5];  // index of the window's last point
+#pragma unroll 4
+  for (int iv = 0; iv < ND; iv++) {
+    double s = c[0] * w0;
+    s += c[1] * w1;
+    s += c[2] * w2;
+    s += c[3] * w3;
+    s += c[4] * w4;
+    s += c[5] * w5;
+    dst[(size_t)iv * sD] = s;
+    nxt = nxt + 1 == ND ? 0 : nxt + 1;
+    w0 = w1;
+    w1 = w2;
+    w2 = w3;
+    w3 = w4;
+    w4 = w5;
+    w5 = src[(size_t)nxt * sD];
   }
+}
+
+// x/2 then y/2 through one LDS-resident (x, y) plane (the first and last
+// pair of VO/timestep.hpp:11-55 in one pass over f).  Workgroup of 1024 lanes
+// per (vy, run of G vx planes).  Per plane: stage it in LDS from registers
+// (its loads were issued before the previous plane's y pass); x pass -- wave w
+// owns rows w, w + 16, ...: it reads the stencils of a whole row, then writes
+// the row back in place (no other wave touches it, and one wave's LDS
+// operations execute in order); y pass from LDS to HBM.  The weights come
+// from the context's tables (k_lag_tables): the x ones (x, vx) into registers
+// once per plane -- a lane's x positions are fixed -- and the y ones (y, vy)
+// as wave-uniform reads when a wave covers part of one row; the LDS holds
+// only the plane.  Each cell's arithmetic is that of k_lag_x followed by
+// k_lag_y.
+constexpr int LAG_XY_THREADS = 1024;
+constexpr int LAG_XY_MAXM = 16;  // plane points per lane: planes up to 16384 points
+
+struct LagTab {  // stencil of one point: six weights and the wrapped first index
+  double w[6];
+  int b;
+};
+static_assert(sizeof(LagTab) == 7 * sizeof(double), "LagTab is copied as 7 doubles");
+
+__device__ __forceinline__ void lag_tab(const Geom &g, int d, double h, double dep, LagTab &t) {
+  int j[6];
+  lag_stencil(g, d, h, dep, j, t.w);
+  t.b = j[0];
+}
+
+// The stencils of the x/2 and y/2 passes, fixed for a context: x for every
+// (vx, x) -- N2 x N0 entries -- then y for every (vy, y) -- N3 x N1.
+__global__ __launch_bounds__(256) void k_lag_tables(LagTab *__restrict__ tab, Geom g, double dt) {
+  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N2 * N0) {
+    const int ivx = i / N0, ix = i % N0;
+    lag_tab(g, 0, g.minPhy[0] + ix * g.dx[0], dt * (g.minPhy[2] + ivx * g.dx[2]), tab[i]);
+  } else if (i < N2 * N0 + N3 * N1) {
+    const int k = i - N2 * N0, ivy = k / N1, iy = k % N1;
+    lag_tab(g, 1, g.minPhy[1] + iy * g.dx[1], dt * (g.minPhy[3] + ivy * g.dx[3]), tab[i]);
+  }
+}
+
+template <int NC>
+__global__ __launch_bounds__(LAG_XY_THREADS) void k_lag_xy(const double *__restrict__ in, double *__restrict__ out,
+                                                           Geom g, const LagTab *__restrict__ tab, int G,
+                                                           int nchunk) {
+  extern __shared__ double pl[];  // the (x, y) plane, N0 * N1 doubles
+  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2];
+  const int P = N0 * N1;
+  const int ivy = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
+  const int v0 = chunk * G, v1 = min(N2, v0 + G);
+  const int l = threadIdx.x, wave = l >> 6, lane = l & 63;
+  constexpr int NW = LAG_XY_THREADS / 64;
+  const LagTab *ytab = tab + (size_t)N2 * N0 + (size_t)ivy * N1;
+  // with N0 a multiple of 64 every wave of the y pass covers part of one row:
+  // its weights are wave-uniform
+  const bool row_waves = (N0 & 63) == 0;
+  const int dix = LAG_XY_THREADS % N0, diy = LAG_XY_THREADS / N0;
+  // the plane in flight: loaded during the previous plane's y pass
+  double r[LAG_XY_MAXM];
+  {
+    const double *src = in + ((size_t)ivy * N2 + v0) * (size_t)P;
+#pragma unroll
+    for (int m = 0; m < LAG_XY_MAXM; m++) {
+      const int p = l + LAG_XY_THREADS * m;
+      r[m] = p < P ? src[p] : 0.;
+    }
+  }
+  for (int ivx = v0; ivx < v1; ivx++) {
+    const size_t off = ((size_t)ivy * N2 + ivx) * (size_t)P;
+    // this lane's x stencils (x = lane + 64 c) for this vx
+    LagTab xr[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const int ix = lane + 64 * c;
+      if (ix < N0) xr[c] = tab[(size_t)ivx * N0 + ix];
+    }
+    __syncthreads();  // the previous plane's y pass has read pl
+#pragma unroll
+    for (int m = 0; m < LAG_XY_MAXM; m++) {
+      const int p = l + LAG_XY_THREADS * m;
+      if (p < P) pl[p] = r[m];
+    }
+    __syncthreads();
+    for (int row = wave; row < N1; row += NW) {
+      double *pr = pl + row * N0;
+      double xv[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const int ix = lane + 64 * c;
+        xv[c] = 0.;
+        if (ix < N0) {
+          double s = 0.;
+#pragma unroll
+          for (int k = 0; k < 6; k++) {
+            int q = xr[c].b + k;
+            q = q >= N0 ? q - N0 : q;
+            s += xr[c].w[k] * pr[q];
+          }
+          xv[c] = s;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const int ix = lane + 64 * c;
+        if (ix < N0) pr[ix] = xv[c];
+      }
+    }
+    __syncthreads();
+    if (ivx + 1 < v1) {  // next plane's loads, in flight during the y pass
+      const double *nx = in + off + (size_t)P;
+#pragma unroll
+      for (int m = 0; m < LAG_XY_MAXM; m++) {
+        const int p = l + LAG_XY_THREADS * m;
+        if (p < P) r[m] = nx[p];
+      }
+    }
+    double *dst = out + off;
+    int ix = l % N0, iy = l / N0;
+#pragma unroll 4
+    for (int p = l; p < P; p += LAG_XY_THREADS) {
+      const int ty = row_waves ? __builtin_amdgcn_readfirstlane(iy) : iy;
+      const LagTab &t = ytab[ty];
+      double s = 0.;
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        int q = t.b + k;
+        q = q >= N1 ? q - N1 : q;
+        s += t.w[k] * pl[q * N0 + ix];
+      }
+      dst[p] = s;
+      ix += dix;
+      iy += diy;
+      if (ix >= N0) {
+        ix -= N0;
+        iy++;
+      }
+    }
+  }
+}
+
+static size_t lag_xy_lds(const Geom &g) { return (size_t)g.n[0] * g.n[1] * sizeof(double); }
+
+// vx runs per vy of the fused x/y pass (0: the plane does not fit; separate
+// kernels).  About 1024 workgroups; VLP4D_LAG_FUSED=0 disables the fusion.
+int lag_xy_chunks(const Geom &g) {
+  if (const char *e = getenv("VLP4D_LAG_FUSED"))
+    if (atoi(e) == 0) return 0;
+  // rows of up to 128 points (two 64-lane chunks of x weights in registers)
+  if (g.n[0] > 128 || (long)g.n[0] * g.n[1] > (long)LAG_XY_THREADS * LAG_XY_MAXM || lag_xy_lds(g) > 160 * 1024)
+    return 0;
+  const int want = (1024 + g.n[3] - 1) / g.n[3];
+  const int G = (g.n[2] + want - 1) / want;  // vx planes per workgroup
+  return (g.n[2] + G - 1) / G;
 }
 
 // Initial conditions of VO/init.cpp:144-285 (TSI20 differs from the spline
@@ -195,13 +369,41 @@ void launch_lag_x(const Geom &g, const double *in, double *out, double dt, hipSt
   hipLaunchKernelGGL(k_lag_x, dim3(blocks(n)), dim3(256), 0, s, in, out, g, dt);
 }
 
+size_t lag_tables_bytes(const Geom &g) {
+  return ((size_t)g.n[2] * g.n[0] + (size_t)g.n[3] * g.n[1]) * sizeof(LagTab);
+}
+
+void launch_lag_tables(const Geom &g, void *tab, double dt, hipStream_t s) {
+  const int n = g.n[2] * g.n[0] + g.n[3] * g.n[1];
+  hipLaunchKernelGGL(k_lag_tables, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (LagTab *)tab, g, dt);
+}
+
+bool launch_lag_xy(const Geom &g, const double *in, double *out, const void *tab, int nchunk, hipStream_t s) {
+  if (nchunk <= 0 || !tab) return false;
+  const int G = (g.n[2] + nchunk - 1) / nchunk;
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void *)k_lag_xy<1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void *)k_lag_xy<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  const dim3 grid((unsigned)(g.n[3] * nchunk)), block(LAG_XY_THREADS);
+  const size_t lds = lag_xy_lds(g);
+  const LagTab *t = (const LagTab *)tab;
+  if (g.n[0] <= 64)  // 64-point chunks of an x row
+    hipLaunchKernelGGL(k_lag_xy<1>, grid, block, lds, s, in, out, g, t, G, nchunk);
+  else
+    hipLaunchKernelGGL(k_lag_xy<2>, grid, block, lds, s, in, out, g, t, G, nchunk);
+  return true;
+}
+
 void launch_lag_y(const Geom &g, const double *in, double *out, double dt, double *rho_part, hipStream_t s) {
   const long n = (long)g.n[0] * g.n[1] * g.n[3];
   hipLaunchKernelGGL(k_lag_y, dim3(blocks(n)), dim3(256), 0, s, in, out, g, dt, rho_part);
 }
 
 void launch_lag_v(const Geom &g, int d, const double *in, double *out, const double *e, double dt, hipStream_t s) {
-  const long n = (long)g.n[0] * g.n[1] * g.n[d];
+  const long n = (long)g.n[0] * g.n[1] * g.n[d == 2 ? 3 : 2];
   if (d == 2) hipLaunchKernelGGL(k_lag_v<2>, dim3(blocks(n)), dim3(256), 0, s, in, out, g, e, dt);
   else hipLaunchKernelGGL(k_lag_v<3>, dim3(blocks(n)), dim3(256), 0, s, in, out, g, e, dt);
 }

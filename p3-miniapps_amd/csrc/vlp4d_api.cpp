@@ -34,6 +34,7 @@ struct vlp_ctx {
   Geom g{};
   int precision = VLP_FP64;
   int scheme = VLP_SCHEME_SPLINE;  // VLP_SCHEME_LAGRANGE: the split-step Lagrange solver (lagrange.hip)
+  void *lag_tab = nullptr;          // stencil tables of its fused x/2, y/2 pass
   int device = 0;
   hipStream_t stream = nullptr;
   size_t esz = 8;
@@ -136,7 +137,7 @@ void free_all(vlp_ctx *c) {
                   c->fb.rho,   c->fb.ex,    c->fb.ey,     c->fb.exy, c->fb.spec1, c->fb.spec2, c->fb.twx,  c->fb.twy,
                   c->fb.diag_part, c->fb.rho_tmp, c->fb.diag, c->d_err, c->adj_c4s, c->adj_blk, c->adj_tmp,
                   c->adj_cxy, c->adj_E, c->adj_rho, c->adj_k, c->adj_g, c->adj_grad, c->adj_Mt[0], c->adj_Mt[1],
-                  c->adj_Mt[2], c->adj_Mt[3]};
+                  c->adj_Mt[2], c->adj_Mt[3], c->lag_tab};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_err) (void)hipHostFree(c->h_err);
@@ -311,19 +312,33 @@ int lag_step_once(vlp_ctx *c) {
   double *f = (double *)c->f, *t = (double *)c->fnext;
   hipStream_t s = c->stream;
   HIP_TRY(c, hipEventRecord(ev[0], s));
-  vlp::launch_lag_x(g, f, t, 0.5 * dt, s);
-  vlp::launch_lag_y(g, t, f, 0.5 * dt, c->rho_part, s);
-  c->rho_outer = g.n[3];
+  // x/2 + y/2: one pass through an LDS plane and the density from a
+  // streaming pass over its output, or two kernels with the density fused
+  const int nch = vlp::lag_xy_chunks(g);
+  const bool fused = vlp::launch_lag_xy(g, f, t, c->lag_tab, nch, s);
+  if (fused) {
+    std::swap(f, t);
+  } else {
+    vlp::launch_lag_x(g, f, t, 0.5 * dt, s);
+    vlp::launch_lag_y(g, t, f, 0.5 * dt, c->rho_part, s);
+  }
   HIP_TRY(c, hipEventRecord(ev[1], s));
+  if (fused) vlp::launch_moments<double>(g, f, c->rho_part, c->l2_part, s);
+  c->rho_outer = g.n[3];
   int rc = field(c, c->iter + 1, true, false);
   if (rc) return rc;
   HIP_TRY(c, hipEventRecord(ev[2], s));
   vlp::launch_lag_v(g, 2, f, t, c->fb.ex, dt, s);
   vlp::launch_lag_v(g, 3, t, f, c->fb.ey, dt, s);
   HIP_TRY(c, hipEventRecord(ev[3], s));
-  vlp::launch_lag_x(g, f, t, 0.5 * dt, s);
-  vlp::launch_lag_y(g, t, f, 0.5 * dt, nullptr, s);
+  if (vlp::launch_lag_xy(g, f, t, c->lag_tab, nch, s)) {
+    std::swap(f, t);
+  } else {
+    vlp::launch_lag_x(g, f, t, 0.5 * dt, s);
+    vlp::launch_lag_y(g, t, f, 0.5 * dt, nullptr, s);
+  }
   HIP_TRY(c, hipEventRecord(ev[4], s));
+  if (f != (double *)c->f) std::swap(c->f, c->fnext);  // the result is in f
   rc = check_launch(c, "step");
   if (rc) return rc;
   c->iter++;
@@ -528,6 +543,7 @@ static int create_ctx(const vlp_config *cfg_in, int precision, int device, int s
     if (hipEventCreate(&e) != hipSuccess) return fail(set_err(nullptr, VLP_E_HIP, "hipEventCreate failed"));
   ALLOC(c->f, c->n_f * c->esz);
   ALLOC(c->fnext, c->n_f * c->esz);
+  if (lag) ALLOC(c->lag_tab, vlp::lag_tables_bytes(c->g));
   if (!lag) {
     ALLOC(c->cxy, c->n_cxy * c->esz);
     ALLOC(c->tv, c->n_tv * c->esz);
@@ -564,7 +580,10 @@ static int create_ctx(const vlp_config *cfg_in, int precision, int device, int s
     return fail(set_err(nullptr, VLP_E_HIP, "device initialisation failed"));
   // init(): VMO/Init.cpp:370-372 then VMO/Vlp4d.cpp:30-34
   const auto t0 = std::chrono::steady_clock::now();
-  if (lag) vlp::launch_lag_init(c->g, (double *)c->f, c->stream);  // VO/init.cpp:144-285
+  if (lag) {
+    vlp::launch_lag_init(c->g, (double *)c->f, c->stream);  // VO/init.cpp:144-285
+    vlp::launch_lag_tables(c->g, c->lag_tab, 0.5 * cfg.dt, c->stream);
+  }
   else if (precision == VLP_FP64) vlp::launch_init_case<double>(c->g, (double *)c->f, c->stream);
   else vlp::launch_init_case<float>(c->g, (float *)c->f, c->stream);
   rc = refresh_from_f(c);
@@ -773,7 +792,7 @@ const char *vlp_last_error(const vlp_ctx *c) { return c ? c->err.c_str() : g_las
 
 int vlp_kernel_variants(const vlp_ctx *c) {
   if (!c) return -1;
-  if (c->scheme == VLP_SCHEME_LAGRANGE) return 0;
+  if (c->scheme == VLP_SCHEME_LAGRANGE) return vlp::lag_xy_chunks(c->g) > 0 ? 1 : 0;  // bit 0: fused x/y pass
   int bits = vlp::kernel_variants(c->g, c->precision != VLP_FP64, c->precision == VLP_FP32_ACC);
   if ((bits & 128) && c->ys_tvx) bits = (bits & ~(15 << 8)) | (c->ys_tvx << 8);
   return bits;

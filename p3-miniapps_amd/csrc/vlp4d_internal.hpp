@@ -96,6 +96,15 @@ void launch_lag_init(const Geom &g, double *f, hipStream_t s);
 void launch_lag_x(const Geom &g, const double *in, double *out, double dt, hipStream_t s);
 void launch_lag_y(const Geom &g, const double *in, double *out, double dt, double *rho_part, hipStream_t s);
 void launch_lag_v(const Geom &g, int d, const double *in, double *out, const double *e, double dt, hipStream_t s);
+// Fused x/2 + y/2 (one pass over f) when the (x, y) plane fits in LDS:
+// lag_xy_chunks gives the vx runs per vy, 0 when not eligible;
+// launch_lag_xy returns false for nchunk 0.
+// The stencil tables of the x/2, y/2 passes (lag_tables_bytes) are built once
+// per context for the half step dt.
+int lag_xy_chunks(const Geom &g);
+size_t lag_tables_bytes(const Geom &g);
+void launch_lag_tables(const Geom &g, void *tab, double dt, hipStream_t s);
+bool launch_lag_xy(const Geom &g, const double *in, double *out, const void *tab, int nchunk, hipStream_t s);
 
 void launch_adj_advect4d(const Geom &g, const double *c4, const double *ex, const double *ey, double dt,
                          const double *gin, double *lc4, double *lE, bool field_term, hipStream_t s);

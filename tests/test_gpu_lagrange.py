@@ -108,3 +108,23 @@ def test_lagrange_64_conserves_the_total(vlp):
     print(f"Lagrange 64^4 x{steps}: total {s0:.15e} -> {s1:.15e}")
     assert abs(s1 - s0) < 1e-12 * abs(s0)
     assert np.all(np.isfinite(d["nrj"])) and np.max(np.abs(d["mass"])) < 1e-10
+
+
+def test_lagrange_fused_and_separate_xy_passes_agree(vlp, orc, monkeypatch):
+    # the x/2 + y/2 pair through one LDS plane (default where the plane fits)
+    # and as two kernels (VLP4D_LAG_FUSED=0): same cells, same arithmetic;
+    # the density is summed in another order
+    n, idcase, vmax, lx, dt, steps = CASES[1]
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("VLP4D_LAG_FUSED", fused)
+        sim, st = _pair(vlp, orc, n, idcase, vmax, lx, dt, steps)
+        try:
+            sim.step(steps)
+            st.step(steps)
+            assert relmax(sim.get_f(), st.f()) < 1e-12
+            out[fused] = sim.get_f()
+        finally:
+            sim.close()
+            st.close()
+    assert relmax(out["1"], out["0"]) < 1e-13
