@@ -11,7 +11,10 @@
 // Extra flags: "--fp32" selects FP32 storage of f and the spline
 // coefficients (double interpolation arithmetic), "--fp32-acc" FP32 storage
 // with float interpolation arithmetic, "-d N" the device (as the reference's
-// --device).
+// --device).  "--lagrange" runs the Lagrange split-step solver of the non-MPI
+// miniapp instead (miniapps/vlp4d/openmp/vlp4d.cpp:33-81: same deck and echo,
+// its 3-column nrj.out, no l2norm.out); that executable takes the deck as its
+// only argument, so a bare path is read as the deck too.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -49,11 +52,14 @@ static void print_config(const vlp_config *c) {
 int main(int argc, char **argv) {
   const char *file = "data.dat";
   int precision = VLP_FP64, device = 0;
+  bool lagrange = false;
   for (int i = 1; i < argc; i++) {
     if ((!strcmp(argv[i], "-f") || !strcmp(argv[i], "--file")) && i + 1 < argc) file = argv[++i];
     else if ((!strcmp(argv[i], "-d") || !strcmp(argv[i], "--device")) && i + 1 < argc) device = atoi(argv[++i]);
     else if (!strcmp(argv[i], "--fp32")) precision = VLP_FP32;
     else if (!strcmp(argv[i], "--fp32-acc")) precision = VLP_FP32_ACC;
+    else if (!strcmp(argv[i], "--lagrange")) lagrange = true;
+    else if (argv[i][0] != '-') file = argv[i];
   }
   printf("reading input file %s\n", file);
   vlp_config cfg;
@@ -65,7 +71,7 @@ int main(int argc, char **argv) {
   }
   print_config(&cfg);
   vlp_ctx *ctx = nullptr;
-  rc = vlp_create_config(&cfg, precision, device, &ctx);
+  rc = lagrange ? vlp_create_lagrange(&cfg, device, &ctx) : vlp_create_config(&cfg, precision, device, &ctx);
   if (rc == VLP_E_CASE) {
     printf("Unknown test case !\n");
     fflush(stdout);
@@ -75,9 +81,10 @@ int main(int argc, char **argv) {
     fprintf(stderr, "vlp4d: %s\n", vlp_last_error(nullptr));
     return 1;
   }
-  printf("[0] local domain [%4u:%4u,%4u:%4u,%4u:%4u,%4u:%4u] cost %lu\n", 0u, cfg.nxmax[0] - 1, 0u, cfg.nxmax[1] - 1,
-         0u, cfg.nxmax[2] - 1, 0u, cfg.nxmax[3] - 1,
-         (unsigned long)cfg.nxmax[0] * cfg.nxmax[1] * cfg.nxmax[2] * cfg.nxmax[3]);
+  if (!lagrange)  // the domain line of the MPI miniapp's decomposition
+    printf("[0] local domain [%4u:%4u,%4u:%4u,%4u:%4u,%4u:%4u] cost %lu\n", 0u, cfg.nxmax[0] - 1, 0u, cfg.nxmax[1] - 1,
+           0u, cfg.nxmax[2] - 1, 0u, cfg.nxmax[3] - 1,
+           (unsigned long)cfg.nxmax[0] * cfg.nxmax[1] * cfg.nxmax[2] * cfg.nxmax[3]);
   while (vlp_iter(ctx) < cfg.nbiter) {
     printf("iter %d\n", vlp_iter(ctx));
     rc = vlp_step(ctx, 1);

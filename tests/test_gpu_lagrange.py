@@ -128,3 +128,26 @@ def test_lagrange_fused_and_separate_xy_passes_agree(vlp, orc, monkeypatch):
             sim.close()
             st.close()
     assert relmax(out["1"], out["0"]) < 1e-13
+
+
+def test_lagrange_driver_matches_oracle(orc, tmp_path):
+    # the executable with --lagrange and the deck as its only positional
+    # argument (the non-MPI miniapp's surface), against the oracle's series
+    import subprocess
+    from conftest import DECKS, ROOT
+    txt = open(os.path.join(DECKS, "SLD10.dat")).read().replace(": 128", ": 16")
+    txt = txt.replace("Time step : 0.01", "Time step : 0.1").replace("iterations : 40", "iterations : 4")
+    (tmp_path / "d.dat").write_text(txt)
+    p = subprocess.run([os.path.join(ROOT, "p3-miniapps_amd", "vlp4d"), "--lagrange", "d.dat"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert [ln for ln in p.stdout.splitlines() if ln.startswith("iter ")] == [f"iter {i}" for i in range(4)]
+    assert "local domain" not in p.stdout
+    rows = np.array([[float(v) for v in ln.split()] for ln in (tmp_path / "nrj.out").read_text().splitlines()])
+    assert rows.shape == (5, 3) and not (tmp_path / "l2norm.out").exists()
+    st = orc.LagState(orc.parse_deck(str(tmp_path / "d.dat")))
+    st.step(4)
+    ref = st.diags()
+    st.close()
+    np.testing.assert_allclose(rows[:, 1], ref["nrj"], rtol=1e-11)
+    np.testing.assert_allclose(rows[:, 0], np.arange(5) * 0.1, rtol=1e-12)
