@@ -205,6 +205,14 @@ int vlp_tuned_tile(const vlp_ctx *ctx, int *vx_tile, int *plane_buffers);
  * It has no counterpart in the reference, which has no derivative code; it
  * returns VLP_E_CFL where the step from this f would. */
 int vlp_adjoint_step(vlp_ctx *ctx, const double *g, double *grad, size_t n);
+/* Adjoint of nsteps steps from the context's current f (VLP_FP64):
+ * grad = (d f^{n+nsteps} / d f^n)^T g, by two-level checkpointing (every
+ * ceil(sqrt(nsteps))-th state kept on the device, the others recomputed in
+ * the reverse sweep; about 2 sqrt(nsteps) extra copies of f).  Needs iter +
+ * nsteps <= nbiter; the context ends at its starting f and iteration (the
+ * recomputed steps write the diagnostics a run would write).  No counterpart
+ * in the reference. */
+int vlp_adjoint_steps(vlp_ctx *ctx, int nsteps, const double *g, double *grad, size_t n);
 
 /* Ensembles: independent contexts (any grid sizes and precisions, one
  * device) advanced together, two steps of every member per launch of one

@@ -35,7 +35,7 @@ EXPORTS = [
     "vlp_buffer_size", "vlp_get_buffer", "vlp_spline_xy", "vlp_advect2d_poisson",
     "vlp_spline_vxvy", "vlp_advect4d_poisson", "vlp_timer", "vlp_timer_label", "vlp_last_error",
     "vlp_kernel_variants", "vlp_autotune", "vlp_tuned_tile", "vlp_ensemble_create", "vlp_ensemble_step",
-    "vlp_ensemble_size", "vlp_ensemble_destroy", "vlp_adjoint_step", "vlp_create_lagrange",
+    "vlp_ensemble_size", "vlp_ensemble_destroy", "vlp_adjoint_step", "vlp_adjoint_steps", "vlp_create_lagrange",
 ]
 
 
@@ -121,6 +121,7 @@ def load(path=LIB_PATH):
     lib.vlp_ensemble_size.argtypes = [vp]
     lib.vlp_ensemble_destroy.argtypes = [vp]
     lib.vlp_adjoint_step.argtypes = [vp, dp, dp, ctypes.c_size_t]
+    lib.vlp_adjoint_steps.argtypes = [vp, ctypes.c_int, dp, dp, ctypes.c_size_t]
     lib.vlp_create_lagrange.argtypes = [P(VlpConfig), ctypes.c_int, P(vp)]
     _lib = lib
     return lib
@@ -225,6 +226,15 @@ class Vlp4d:
         a = np.ascontiguousarray(g, dtype=np.float64).ravel()
         out = np.empty_like(a)
         self._check(self.lib.vlp_adjoint_step(self.h, _dptr(a), _dptr(out), a.size))
+        return out.reshape(self.shape4[::-1])
+
+    def adjoint_steps(self, nsteps, g):
+        """Vector-Jacobian product of nsteps steps from the current f (FP64
+        contexts, checkpointed reverse sweep); the context ends where it
+        started."""
+        a = np.ascontiguousarray(g, dtype=np.float64).ravel()
+        out = np.empty_like(a)
+        self._check(self.lib.vlp_adjoint_steps(self.h, int(nsteps), _dptr(a), _dptr(out), a.size))
         return out.reshape(self.shape4[::-1])
 
     def buffer(self, which):

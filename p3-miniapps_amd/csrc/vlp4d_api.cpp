@@ -13,6 +13,7 @@
 // The reference's deep copies, transposes and halo buffers have no counterpart.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -869,11 +870,10 @@ int vlp_autotune(vlp_ctx *c, const char *csv_path) {
 // is then C4^T I4^T g plus the foot term through E, K^T and the half step
 // (adjoint.hip).  The coefficient, density and field buffers are left
 // holding intermediate values; the next step recomputes them from f.
-int vlp_adjoint_step(vlp_ctx *c, const double *g_host, double *grad_host, size_t n) {
-  int rc = spline_only(c);
-  if (rc) return rc;
-  if (c->precision != VLP_FP64) return set_err(c, VLP_E_ARG, "the adjoint is implemented for VLP_FP64 contexts");
-  if (!g_host || !grad_host || n != c->n_f) return set_err(c, VLP_E_SIZE, "f buffer size mismatch");
+// Buffers of the adjoint, the line operators M_d^T and the field's response
+// K (built on first use).
+static int adjoint_prepare(vlp_ctx *c) {
+  int rc = VLP_OK;
   const Geom &g = c->g;
   const long N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
   const long E0 = N0 + 4, E1 = N1 + 4, E2 = N2 + 4, E3 = N3 + 4;
@@ -931,13 +931,22 @@ int vlp_adjoint_step(vlp_ctx *c, const double *g_host, double *grad_host, size_t
     HIP_TRY(c, hipMemcpyAsync(c->adj_k + N0 * N1, c->fb.ey, N0 * N1 * sizeof(double), hipMemcpyDeviceToDevice, s));
     HIP_TRY(c, hipStreamSynchronize(s));
   }
+  return VLP_OK;
+}
+
+// (d f^{n+1}/d f^n)^T adj_g -> adj_grad at the context's f, on its stream.
+static int adjoint_core(vlp_ctx *c) {
+  const Geom &g = c->g;
+  const long N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const long E0 = N0 + 4, E1 = N1 + 4, E2 = N2 + 4, E3 = N3 + 4;
+  hipStream_t s = c->stream;
+  const bool field_term = g.idcase != 2 && g.idcase != 6;
   // forward phases of the step from f (f itself untouched)
   run_spline_xy<double>(c);
   run_advect2d<double>(c);
-  rc = field(c, c->iter + 1, false, false);
+  int rc = field(c, c->iter + 1, false, false);
   if (rc) return rc;
   run_spline_vxvy<double>(c);
-  HIP_TRY(c, hipMemcpyAsync(c->adj_g, g_host, c->n_f * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(c, hipMemsetAsync(c->adj_c4s, 0, c->n_c4s * sizeof(double), s));
   HIP_TRY(c, hipMemsetAsync(c->adj_E, 0, 2 * N0 * N1 * sizeof(double), s));
   vlp::launch_adj_advect4d(g, (const double *)c->c4, c->fb.ex, c->fb.ey, c->cfg.dt, c->adj_g, c->adj_c4s, c->adj_E,
@@ -956,10 +965,90 @@ int vlp_adjoint_step(vlp_ctx *c, const double *g_host, double *grad_host, size_t
     vlp::launch_axis_contract(c->adj_Mt[1], (int)E1, (int)N1, c->adj_cxy, c->adj_tmp, N2 * N3, E0, false, s);
     vlp::launch_axis_contract(c->adj_Mt[0], (int)E0, (int)N0, c->adj_tmp, c->adj_grad, N1 * N2 * N3, 1, true, s);
   }
-  rc = check_launch(c, "adjoint");
+  return check_launch(c, "adjoint");
+}
+
+static int adjoint_args(vlp_ctx *c, const double *g_host, double *grad_host, size_t n) {
+  int rc = spline_only(c);
   if (rc) return rc;
-  HIP_TRY(c, hipMemcpyAsync(grad_host, c->adj_grad, c->n_f * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (c->precision != VLP_FP64) return set_err(c, VLP_E_ARG, "the adjoint is implemented for VLP_FP64 contexts");
+  if (!g_host || !grad_host || n != c->n_f) return set_err(c, VLP_E_SIZE, "f buffer size mismatch");
+  return adjoint_prepare(c);
+}
+
+int vlp_adjoint_step(vlp_ctx *c, const double *g_host, double *grad_host, size_t n) {
+  int rc = adjoint_args(c, g_host, grad_host, n);
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(c->adj_g, g_host, c->n_f * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  rc = adjoint_core(c);
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(grad_host, c->adj_grad, c->n_f * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   return sync_and_check(c);  // the recomputed phases flag a foot out of the box
+}
+
+// Adjoint of nsteps steps from the context's f: grad = (d f^{n+K}/d f^n)^T g
+// by two-level checkpointing -- the forward sweep keeps every C-th state (C =
+// ceil(sqrt(K))); the reverse sweep recomputes each segment's states from its
+// checkpoint and applies the one-step adjoint backwards through them.  Device
+// memory: about 2 sqrt(K) copies of f.  The context ends at its starting f and
+// iteration; the diagnostics of the recomputed steps are written as a run
+// would write them.
+int vlp_adjoint_steps(vlp_ctx *c, int nsteps, const double *g_host, double *grad_host, size_t n) {
+  int rc = adjoint_args(c, g_host, grad_host, n);
+  if (rc) return rc;
+  if (nsteps < 1 || c->iter + nsteps > c->cfg.nbiter)
+    return set_err(c, VLP_E_ARG, "nsteps must be at least 1 and within nbiter");
+  hipStream_t s = c->stream;
+  const size_t fb = c->n_f * sizeof(double);
+  const int K = nsteps, C = (int)std::ceil(std::sqrt((double)K)), nck = (K + C - 1) / C;
+  std::vector<double *> ck(nck, nullptr), seg(C, nullptr);
+  auto release = [&]() {
+    for (double *p : ck)
+      if (p) (void)hipFree(p);
+    for (double *p : seg)
+      if (p) (void)hipFree(p);
+  };
+  for (auto *v : {&ck, &seg})
+    for (double *&p : *v)
+      if (hipMalloc((void **)&p, fb) != hipSuccess) {
+        release();
+        return set_err(c, VLP_E_HIP, "hipMalloc failed (adjoint checkpoints)");
+      }
+  const int iter0 = c->iter;
+  auto restore = [&](const double *src, int it) -> int {
+    HIP_TRY(c, hipMemcpyAsync(c->f, src, fb, hipMemcpyDeviceToDevice, s));
+    c->iter = it;
+    return VLP_OK;
+  };
+  // forward sweep: checkpoints at 0, C, 2C, ...
+  for (int k = 0; k < K && !rc; k++) {
+    if (k % C == 0) {
+      if (hipMemcpyAsync(ck[k / C], c->f, fb, hipMemcpyDeviceToDevice, s) != hipSuccess) rc = VLP_E_HIP;
+    }
+    if (!rc) rc = step_once(c);
+  }
+  if (!rc && hipMemcpyAsync(c->adj_g, g_host, fb, hipMemcpyHostToDevice, s) != hipSuccess) rc = VLP_E_HIP;
+  // reverse sweep, segment by segment
+  for (int sg = nck - 1; sg >= 0 && !rc; sg--) {
+    const int k0 = sg * C, len = std::min(C, K - k0);
+    rc = restore(ck[sg], iter0 + k0);
+    for (int j = 0; j < len && !rc; j++) {  // the segment's states
+      if (hipMemcpyAsync(seg[j], c->f, fb, hipMemcpyDeviceToDevice, s) != hipSuccess) rc = VLP_E_HIP;
+      if (!rc && j + 1 < len) rc = step_once(c);
+    }
+    for (int j = len - 1; j >= 0 && !rc; j--) {
+      rc = restore(seg[j], iter0 + k0 + j);
+      if (!rc) rc = adjoint_core(c);
+      if (!rc) std::swap(c->adj_g, c->adj_grad);  // the result is the next cotangent
+    }
+  }
+  if (!rc) rc = restore(ck[0], iter0);
+  if (!rc && hipMemcpyAsync(grad_host, c->adj_g, fb, hipMemcpyDeviceToHost, s) != hipSuccess) rc = VLP_E_HIP;
+  if (!rc) rc = sync_and_check(c);
+  else (void)hipStreamSynchronize(s);
+  release();
+  if (rc == VLP_E_HIP && c->err.empty()) return set_err(c, VLP_E_HIP, "adjoint sweep failed");
+  return rc;
 }
 
 int vlp_tuned_tile(const vlp_ctx *c, int *vx_tile, int *plane_buffers) {

@@ -169,3 +169,61 @@ def test_adjoint_rejects_bad_requests(vlp):
     with _sim(vlp, n, idcase, vmax, lx) as sim:
         with pytest.raises(vlp.Vlp4dError):
             sim.adjoint_step(np.zeros(10))
+
+
+def test_checkpointed_sweep_matches_chained_steps(vlp):
+    # vlp_adjoint_steps(K) against K one-step adjoints chained by hand from
+    # stored states (a composition independent of the checkpoint schedule);
+    # K = 5 has segments of 3 and 2
+    n, idcase, vmax, lx = NONLINEAR[1]
+    rng = np.random.default_rng(17)
+    with _sim(vlp, n, idcase, vmax, lx, nbiter=20) as sim:
+        sim.step(1)
+        f0 = sim.get_f().copy()
+        g = _off_corner(rng.standard_normal(f0.shape))
+        np.testing.assert_allclose(sim.adjoint_steps(1, g), sim.adjoint_step(g), rtol=0, atol=1e-12 * np.abs(g).max())
+        K = 5
+        states = [f0]
+        for _ in range(K - 1):
+            states.append(_apply(sim, states[-1]))
+        sim.set_f(f0)
+        lam = g
+        for k in range(K - 1, -1, -1):
+            sim.set_f(states[k])
+            lam = sim.adjoint_step(lam)
+        sim.set_f(f0)
+        it = sim.iter
+        got = sim.adjoint_steps(K, g)
+        assert sim.iter == it
+        np.testing.assert_array_equal(sim.get_f(), f0)
+        rel = np.max(np.abs(got - lam)) / np.max(np.abs(lam))
+        print(f"checkpointed {K}-step adjoint vs chained: rel {rel:.2e}")
+        assert rel < 1e-12
+
+
+def test_checkpointed_sweep_matches_finite_differences(vlp):
+    n, idcase, vmax, lx = NONLINEAR[1]
+    rng = np.random.default_rng(23)
+    K = 3
+    with _sim(vlp, n, idcase, vmax, lx, nbiter=60) as sim:
+        sim.step(1)
+        f0 = sim.get_f().copy()
+        it0 = sim.iter
+        g = _off_corner(rng.standard_normal(f0.shape))
+        grad = sim.adjoint_steps(K, g)
+        d = rng.standard_normal(f0.shape)
+        d *= np.max(np.abs(f0)) / np.max(np.abs(d))
+
+        def L(f):
+            sim.set_f(f)
+            sim.step(K)
+            v = float(np.vdot(g, sim.get_f()))
+            return v
+
+        eps = 1e-4
+        fd = (8 * (L(f0 + eps * d) - L(f0 - eps * d)) - (L(f0 + 2 * eps * d) - L(f0 - 2 * eps * d))) / (12 * eps)
+        ad = float(np.vdot(grad, d))
+        rel = abs(fd - ad) / abs(fd)
+        print(f"{K}-step adjoint: finite difference {fd:.12e}, adjoint {ad:.12e}, rel {rel:.2e}")
+        assert rel < 1e-8
+        assert it0 < sim.iter
