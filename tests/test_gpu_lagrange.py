@@ -22,6 +22,10 @@ CASES = [
     ((20, 16, 12, 18), 20, 8.0, 10 * math.pi, 0.1, 6),
     ((12, 14, 16, 10), 2, 1.0, 2.0, 0.05, 5),
     ((14, 12, 10, 16), 5, 1.0, 2.0, 0.05, 5),
+    # rows of 128 and 80 points: two 64-lane chunks of x weights in the fused
+    # pass, y weights wave-uniform (128) or per lane (80)
+    ((128, 12, 10, 12), 10, 6.0, 4 * math.pi, 0.1, 3),
+    ((80, 14, 12, 10), 20, 8.0, 10 * math.pi, 0.1, 3),
 ]
 
 
