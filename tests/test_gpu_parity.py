@@ -2,7 +2,7 @@
 
 Each phase of onetimestep (VMO/Timestep.hpp:34-97) is fed the same input on
 both sides and compared in relative max-norm (max|gpu-oracle| / max|oracle|).
-The bar is the north star's 1e-12 for FP64; phase-level checks use 1e-13
+The bar is 1e-12 (relative max-norm) for FP64; phase-level checks use 1e-13
 (single applications; differences are FMA contraction and summation order).
 The grids are deliberately non-cubic so that a swapped dimension or stride
 cannot pass.

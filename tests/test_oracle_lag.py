@@ -14,7 +14,7 @@ mathematics fixes, independently of its own formulas:
 * every 1-D advection moves whole lines by one shift, so with a partition of
   unity the total of f is conserved to round-off by a full step;
 * rho is dvx*dvy times the velocity sum;
-* the driver's step is the composition x/2, y/2, rho + Poisson, vx, vy, x/2,
+* the oracle's onetimestep is the composition x/2, y/2, rho + Poisson, vx, vy, x/2,
   y/2 of VO/timestep.hpp:11-55, with the diagnostics of the mid-step field.
 The reference's own tests for this miniapp hold no golden vectors.
 """
