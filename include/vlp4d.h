@@ -109,6 +109,15 @@ int vlp_create_config(const vlp_config *cfg, int precision, int device, vlp_ctx 
  * vlp_adjoint_step, ensembles) return VLP_E_ARG on such a context. */
 int vlp_create_lagrange(const vlp_config *cfg, int device, vlp_ctx **out);
 
+/* The fxvx diagnostics of the current iteration (print_fxvx,
+ * VMO/Advection.hpp:500-590; miniapps/vlp4d/openmp/advect.hpp:318-341):
+ * dir/fxvx_itNNNNNN.csv = f(x, y=0, vx, vy=Nvy/2), one row per vx, and for
+ * spline contexts dir/fyvy_itNNNNNN.csv = f(x=0, y, vx=Nvx/2, vy), one row
+ * per vy ("%20.13le" values, ", " between them).  The spline miniapp's own
+ * routine writes zeros (its guard is false at y index 0); these are the
+ * sections it means.  The directory must exist. */
+int vlp_write_fxvx(vlp_ctx *ctx, const char *dir);
+
 /* finalize() of VMO/Init.cpp:375-382 (without the file output; see vlp_save). */
 int vlp_destroy(vlp_ctx *ctx);
 

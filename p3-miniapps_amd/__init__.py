@@ -35,7 +35,7 @@ EXPORTS = [
     "vlp_buffer_size", "vlp_get_buffer", "vlp_spline_xy", "vlp_advect2d_poisson",
     "vlp_spline_vxvy", "vlp_advect4d_poisson", "vlp_timer", "vlp_timer_label", "vlp_last_error",
     "vlp_kernel_variants", "vlp_autotune", "vlp_tuned_tile", "vlp_ensemble_create", "vlp_ensemble_step",
-    "vlp_ensemble_size", "vlp_ensemble_destroy", "vlp_adjoint_step", "vlp_adjoint_steps", "vlp_create_lagrange",
+    "vlp_ensemble_size", "vlp_ensemble_destroy", "vlp_adjoint_step", "vlp_adjoint_steps", "vlp_create_lagrange", "vlp_write_fxvx",
 ]
 
 
@@ -123,6 +123,7 @@ def load(path=LIB_PATH):
     lib.vlp_adjoint_step.argtypes = [vp, dp, dp, ctypes.c_size_t]
     lib.vlp_adjoint_steps.argtypes = [vp, ctypes.c_int, dp, dp, ctypes.c_size_t]
     lib.vlp_create_lagrange.argtypes = [P(VlpConfig), ctypes.c_int, P(vp)]
+    lib.vlp_write_fxvx.argtypes = [vp, ctypes.c_char_p]
     _lib = lib
     return lib
 
@@ -218,6 +219,11 @@ class Vlp4d:
     def set_f(self, f):
         a = np.ascontiguousarray(f, dtype=np.float64).ravel()
         self._check(self.lib.vlp_set_f(self.h, _dptr(a), a.size))
+
+    def write_fxvx(self, directory="."):
+        """fxvx (and, for the spline solver, fyvy) cross sections of the
+        current f as CSV files in an existing directory."""
+        self._check(self.lib.vlp_write_fxvx(self.h, os.fsencode(directory)))
 
     def adjoint_step(self, g):
         """Vector-Jacobian product of one step at the current f (FP64

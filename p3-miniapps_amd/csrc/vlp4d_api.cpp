@@ -675,6 +675,53 @@ int vlp_save(vlp_ctx *c, const char *dir) {
   return VLP_OK;
 }
 
+// Cross sections of f for the fxvx diagnostics, in the CSV layout of
+// print_fxvx (miniapps/vlp4d_mpi/openmp/Advection.hpp:500-590,
+// miniapps/vlp4d/openmp/advect.hpp:318-341): one row per velocity index,
+// "%20.13le, " between values.  fxvx = f(x, y = 0, vx, vy = Nvy/2); spline
+// contexts also write fyvy = f(x = 0, y, vx = Nvx/2, vy).  The spline
+// miniapp's own version writes zeros (its guard `nymin <= iy && iy && nymax`
+// is false at iy = 0) and takes vy = Ny/2; these are the sections it means.
+static int write_csv(vlp_ctx *c, const std::string &path, const std::vector<double> &v, int ncol, int nrow) {
+  FILE *f = fopen(path.c_str(), "w");
+  if (!f) return set_err(c, VLP_E_FILE, "cannot write " + path);
+  for (int r = 0; r < nrow; r++)
+    for (int k = 0; k < ncol; k++) fprintf(f, k == ncol - 1 ? "%20.13le\n" : "%20.13le, ", v[(size_t)r * ncol + k]);
+  fclose(f);
+  return VLP_OK;
+}
+
+int vlp_write_fxvx(vlp_ctx *c, const char *dir) {
+  int rc = usable(c);
+  if (rc) return rc;
+  const size_t N0 = c->cfg.nxmax[0], N1 = c->cfg.nxmax[1], N2 = c->cfg.nxmax[2], N3 = c->cfg.nxmax[3];
+  const size_t e = c->esz;
+  const char *f = (const char *)c->f;
+  std::vector<char> a(N0 * N2 * e), b(N1 * N3 * e);
+  // fxvx: N2 rows of N0 contiguous values, N0*N1 apart
+  HIP_TRY(c, hipMemcpy2DAsync(a.data(), N0 * e, f + (N0 * N1 * N2 * (N3 / 2)) * e, N0 * N1 * e, N0 * e, N2,
+                              hipMemcpyDeviceToHost, c->stream));
+  const bool fyvy = c->scheme == VLP_SCHEME_SPLINE;
+  if (fyvy)  // fyvy: for each vy, N1 values N0 apart
+    for (size_t ivy = 0; ivy < N3; ivy++)
+      HIP_TRY(c, hipMemcpy2DAsync(b.data() + ivy * N1 * e, e, f + (N0 * N1 * (N2 / 2 + N2 * ivy)) * e, N0 * e, e, N1,
+                                  hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  auto widen = [&](const std::vector<char> &raw, size_t n) {
+    std::vector<double> v(n);
+    for (size_t i = 0; i < n; i++)
+      v[i] = e == sizeof(double) ? ((const double *)raw.data())[i] : (double)((const float *)raw.data())[i];
+    return v;
+  };
+  const std::string d = dir ? dir : ".";
+  char name[64];
+  snprintf(name, sizeof name, "/fxvx_it%06d.csv", c->iter);
+  rc = write_csv(c, d + name, widen(a, N0 * N2), (int)N0, (int)N2);
+  if (rc || !fyvy) return rc;
+  snprintf(name, sizeof name, "/fyvy_it%06d.csv", c->iter);
+  return write_csv(c, d + name, widen(b, N1 * N3), (int)N1, (int)N3);
+}
+
 int vlp_get_f(vlp_ctx *c, double *host, size_t n) {
   int rc = usable(c);
   if (rc) return rc;

@@ -15,6 +15,8 @@
 // miniapp instead (miniapps/vlp4d/openmp/vlp4d.cpp:33-81: same deck and echo,
 // its 3-column nrj.out, no l2norm.out); that executable takes the deck as its
 // only argument, so a bare path is read as the deck too.
+#include <sys/stat.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -85,6 +87,18 @@ int main(int argc, char **argv) {
     printf("[0] local domain [%4u:%4u,%4u:%4u,%4u:%4u,%4u:%4u] cost %lu\n", 0u, cfg.nxmax[0] - 1, 0u, cfg.nxmax[1] - 1,
            0u, cfg.nxmax[2] - 1, 0u, cfg.nxmax[3] - 1,
            (unsigned long)cfg.nxmax[0] * cfg.nxmax[1] * cfg.nxmax[2] * cfg.nxmax[3]);
+  // fxvx diagnostics: every ifreq steps (VMO/Timestep.hpp:93-95,
+  // miniapps/vlp4d/openmp/timestep.hpp:49-53), and at iteration 0 for the
+  // non-MPI miniapp (its vlp4d.cpp:63), into data/<miniapp>/
+  const char *fx_dir = lagrange ? "data/vlp4d" : "data/vlp4d_mpi";
+  auto fxvx = [&]() {
+    if (!cfg.fxvx || vlp_iter(ctx) % cfg.ifreq) return;
+    mkdir("data", 0755);
+    mkdir(fx_dir, 0755);
+    printf("print_fxvx %d\n", vlp_iter(ctx));
+    if (vlp_write_fxvx(ctx, fx_dir)) fprintf(stderr, "vlp4d: %s\n", vlp_last_error(ctx));
+  };
+  if (lagrange) fxvx();
   while (vlp_iter(ctx) < cfg.nbiter) {
     printf("iter %d\n", vlp_iter(ctx));
     rc = vlp_step(ctx, 1);
@@ -96,6 +110,7 @@ int main(int argc, char **argv) {
       fprintf(stderr, "vlp4d: %s\n", vlp_last_error(ctx));
       return 1;
     }
+    fxvx();
   }
   rc = vlp_save(ctx, ".");
   if (rc) fprintf(stderr, "vlp4d: %s\n", vlp_last_error(ctx));
