@@ -35,6 +35,7 @@ VALU_PEAK_TFS = 78.6   # FP64 vector (= non-packed FP32) peak of MI355X
 # the contraction, 56 for the 4x4 weights, ~100 for the feet.
 ADVECT4D_FLOP_PER_CELL = 860
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "advect4d_traffic_sld10_128.txt")
+LAG_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lagrange_traffic_sld10_128.txt")
 A100_REF = 2.38e9  # 40*128^4 / 4.51995 s, A100 stdpar MainLoop (docs/vlp4d.md:148,168-171)
 SLD10 = dict(minphy=[0.0, 0.0, -9.0, -9.0], maxphy=[12.56637112, 12.56637112, 9.0, 9.0], idcase=10, dt=0.01)
 
@@ -91,14 +92,13 @@ def advect4d_bytes(n, b):
     return b * ((n + 4) ** 4 + n ** 4)
 
 
-def measured_traffic(kernel, precision, n):
-    """HBM bytes per launch of the 4-D advection kernel from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy-kernel calibration), or None when no summary for this kernel,
-    precision and size exists."""
-    if not os.path.exists(TRAFFIC_FILE):
+def measured_traffic(kernel, precision, n, path=TRAFFIC_FILE):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when no summary for this kernel, precision and size exists."""
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         m = re.match(r"^TRAFFIC kernel=(\w+) precision=(\d+) n=(\d+) bytes_per_launch=([0-9.eE+]+)", line)
         if m and m.group(1) == kernel and int(m.group(2)) == precision and int(m.group(3)) == n:
             return float(m.group(4))
@@ -256,7 +256,7 @@ def lagrange_bench(pkg, n, steps, warmup, with_cpu):
         "config": {"workload": f"vlp4d Lagrange split step (miniapps/vlp4d), SLD10 deck at {n}^4",
                    "grid": [n] * 4, "idcase": 10, "dt": SLD10["dt"]},
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kern, 64, n, LAG_TRAFFIC_FILE),
                      "algorithmic_bytes_per_launch": launch_bytes, "avg_launch_ms": 1e3 * launch_s},
         "step_hbm_fraction": (4 if fused else 6) * launch_bytes / (dt / steps) / 1e9 / HBM_PEAK_GBS,
         "phases_ms": {k: round(1e3 * v, 4) for k, v in ph.items()},
