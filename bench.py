@@ -7,10 +7,15 @@ initial data generated on the GPU.  A step is one onetimestep
 and the diagnostics -- the reference's MainLoop timer scope.
 
 Metric: 4-D cell updates per second = N^4 * steps / wall time of the timed
-steps (inputs resident in HBM).  Prints ONE JSON line.
+steps (inputs resident in HBM).  Prints ONE JSON line.  By default the same
+run times the FP32-storage step (f and coefficients stored in float, float
+interpolation arithmetic, FP64 field solve / density / diagnostics) and the
+plain FP64 step: `value` is the FP32-storage rate, the FP64 rate and the
+speed-up sit beside it, with each mode's roofline and the field-energy
+history of the FP32 run against the FP64 run over the same steps.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision 64|32|33]
-                  [--n 128] [--no-cpu-baseline]
+                  [--n 128] [--no-cpu-baseline]   # --precision: one mode only
   python bench.py --lagrange [--n 128] # the Lagrange split-step solver (miniapps/vlp4d)
   python bench.py --calibrate-copy     # plain copy kernel of known size, for
                                        # scaling rocprofv3 FETCH_SIZE/WRITE_SIZE
@@ -29,7 +34,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 
 METRIC = "4D cell updates/s at 128^4 (FP32-store vs FP64 GPU) + % of HBM roofline"
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TFS = 78.6   # FP64 vector (= non-packed FP32) peak of MI355X
+VALU_PEAK_TFS = 78.6      # FP64 vector peak of MI355X (also its non-packed FP32 rate)
+VALU_PEAK_FP32_TFS = 157.3  # FP32 vector peak, reached only with packed FP32 (disabled in this build)
 # Algorithmic flops of one interp_4D + computeFeet per cell, counted from the
 # reference loops (VMO/Advection.hpp:166-218, 221-296): 336 FMA + 16 MUL in
 # the contraction, 56 for the 4x4 weights, ~100 for the feet.
@@ -121,27 +127,44 @@ def calibrate_copy():
                       "launches": 3, "GBps": 2 * 2 ** 30 / dt / 1e9}))
 
 
-def cpu_baseline(n, steps=1):
-    """Oracle (C/OpenMP port of the reference OpenMP step) on the host cores,
-    one MainLoop step of the same deck.  Bounded sample: `steps` step(s)."""
+def cpu_model():
+    """CPU model and physical cores per socket from lscpu (None if absent)."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None, None
+    model = re.search(r"^Model name:\s*(.+)$", out, re.M)
+    cps = re.search(r"^Core\(s\) per socket:\s*(\d+)", out, re.M)
+    return (model.group(1).strip() if model else None), (int(cps.group(1)) if cps else None)
+
+
+def cpu_baseline(n, warmup=1, steps=3):
+    """Oracle (C/OpenMP port of the reference OpenMP step) on the host cores:
+    `warmup` untimed + `steps` timed MainLoop steps of the same deck, per-step
+    times printed by the oracle driver (-T).  Threads: the physical cores of
+    one socket, at most 16 (this host's share of cores per GPU)."""
     exe = os.path.join(ROOT, "oracle", "vlp4d_oracle")
     if not os.path.exists(exe):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
-    cores = min(16, os.cpu_count() or 1)
+    model, per_socket = cpu_model()
+    cores = min(16, per_socket or os.cpu_count() or 1)
     env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
     with tempfile.TemporaryDirectory() as d:
         deck = os.path.join(d, "bench.dat")
         with open(deck, "w") as fh:
-            fh.write(DECK_TEMPLATE.format(n=n, nbiter=steps))
+            fh.write(DECK_TEMPLATE.format(n=n, nbiter=warmup + steps))
         t0 = time.time()
-        out = subprocess.run([exe, "-f", deck], cwd=d, env=env, capture_output=True, text=True, check=True).stdout
+        out = subprocess.run([exe, "-f", deck, "-T"], cwd=d, env=env, capture_output=True, text=True,
+                             check=True).stdout
         wall = time.time() - t0
-    m = re.search(r"^MainLoop ([0-9.eE+-]+) \[s\], (\d+) calls", out, re.M)
-    sec = float(m.group(1))
-    return {"value": n ** 4 * steps / sec, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{steps} MainLoop step(s) of the same 128^4 SLD10 deck, oracle/vlp4d_oracle "
-                      f"(FP64, OpenMP {cores} threads, {sec:.2f} s/step, {wall:.1f} s incl. init)" if n == 128 else
-                      f"{steps} step(s) at {n}^4, {sec:.2f} s"}
+    per = [float(t) for _, t in re.findall(r"^steptime (\d+) ([0-9.eE+-]+)", out, re.M)]
+    timed = per[warmup:]
+    sec = sum(timed) / len(timed)
+    return {"value": n ** 4 / sec, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "cpu_model": model, "cores_per_socket": per_socket,
+            "sample": f"{warmup} warm-up + {steps} timed MainLoop steps of the same SLD10 deck at {n}^4, "
+                      f"oracle/vlp4d_oracle (FP64, OpenMP {cores} threads, {sec:.2f} s/step, "
+                      f"steps {', '.join(f'{t:.2f}' for t in per)} s, {wall:.1f} s incl. init)"}
 
 
 def ensemble_bench(pkg, steps, seed=20261202):
@@ -267,14 +290,69 @@ def lagrange_bench(pkg, n, steps, warmup, with_cpu):
     return res
 
 
+def run_mode(pkg, n, precision, steps, warmup):
+    """Warm-up then `steps` timed vlp_step calls of the SLD10 deck at n^4 in
+    one precision mode; the dominant kernel's roofline from the library's
+    HIP-event timers (recorded on its own stream around that launch)."""
+    nb = warmup + steps
+    cfg = pkg.VlpConfig.make(n, SLD10["minphy"], SLD10["maxphy"], SLD10["idcase"], SLD10["dt"], nb)
+    prec = {64: pkg.FP64, 32: pkg.FP32, 33: pkg.FP32_ACC}[precision]
+    b = 8 if precision == 64 else 4
+    sim = pkg.Vlp4d(cfg, precision=prec)
+    if warmup:
+        sim.step(warmup)
+    t_before = sim.timers()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    dt = time.perf_counter() - t0
+    t_after = sim.timers()
+    phases = {lab: round(1e3 * (t_after[lab][0] - t_before[lab][0]) / max(1, steps), 4)
+              for lab in ("splinecoeff_xy", "advec2D", "field", "splinecoeff_vxvy", "advec4D")}
+    adv_s = (t_after["advec4D"][0] - t_before["advec4D"][0]) / steps
+    variants = sim.kernel_variants()
+    adv_kernel = "k_advect4d_ys" if variants & 128 else "k_advect4d_blk"
+    adv_bytes = advect4d_bytes(n, b)
+    hbm = adv_bytes / adv_s / 1e9
+    valu = ADVECT4D_FLOP_PER_CELL * n ** 4 / adv_s / 1e12
+    arith32 = precision == 33
+    peak = VALU_PEAK_FP32_TFS if arith32 else VALU_PEAK_TFS
+    d = sim.diags()
+    sim.close()
+    step_bytes = algorithmic_bytes(n, b)
+    return {
+        "value": n ** 4 * steps / dt,
+        "ms_per_step": 1e3 * dt / steps,
+        "dtype": {64: "f64", 32: "f32-storage, f64 arithmetic", 33: "f32-storage, f32 interpolation arithmetic"}[
+            precision],
+        # the 4-D advection binds on VALU issue (~860 flop per cell against
+        # 60 / 30 algorithmic bytes), so the roofline is the vector-ALU one
+        # with the HBM fraction of the same launch beside it
+        "roofline": {"bound": "valu", "kernel": adv_kernel, "achieved": valu, "peak": peak, "unit": "TFLOP/s",
+                     "frac": valu / peak, "flop_per_cell": ADVECT4D_FLOP_PER_CELL, "avg_launch_ms": 1e3 * adv_s,
+                     "peak_note": ("published packed-FP32 vector peak; this build disables packed FP32, whose "
+                                   "non-packed rate is 78.6 TFLOP/s, the frac against that is "
+                                   f"{valu / VALU_PEAK_TFS:.3f}") if arith32 else "FP64 vector peak",
+                     "hbm": {"achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm / HBM_PEAK_GBS,
+                             "algorithmic_bytes_per_launch": adv_bytes,
+                             "traffic": measured_traffic(adv_kernel, precision, n),
+                             "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)}},
+        "step_hbm_fraction": step_bytes / (dt / steps) / 1e9 / HBM_PEAK_GBS,
+        "step_algorithmic_bytes": step_bytes,
+        "phases_ms": phases,
+        "final_nrj": float(d["nrj"][-1]),
+        "nrj": [float(x) for x in d["nrj"]],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", type=int, default=64, choices=[64, 32, 33],
-                    help="64: FP64; 32: FP32 storage, FP64 interpolation arithmetic; "
-                         "33: FP32 storage and FP32 interpolation arithmetic")
+    ap.add_argument("--precision", type=int, default=0, choices=[0, 64, 32, 33],
+                    help="one mode only -- 64: FP64; 32: FP32 storage, FP64 interpolation arithmetic; "
+                         "33: FP32 storage and FP32 interpolation arithmetic.  Default (0): 33 (the value), "
+                         "64 and 32 in the same run")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -298,62 +376,51 @@ def main():
         print(json.dumps(lagrange_bench(pkg, a.n, a.steps, a.warmup, not a.no_cpu_baseline)))
         return
     n = a.n
-    nb = a.warmup + a.steps
-    cfg = pkg.VlpConfig.make(n, SLD10["minphy"], SLD10["maxphy"], SLD10["idcase"], SLD10["dt"], nb)
-    prec = {64: pkg.FP64, 32: pkg.FP32, 33: pkg.FP32_ACC}[a.precision]
-    b = 8 if a.precision == 64 else 4
-    sim = pkg.Vlp4d(cfg, precision=prec)
-    if a.warmup:
-        sim.step(a.warmup)
-    t_before = sim.timers()
-    t0 = time.perf_counter()
-    sim.step(a.steps)
-    t1 = time.perf_counter()
-    t_after = sim.timers()
-    dt = t1 - t0
-    ms = 1e3 * dt / a.steps
-    value = n ** 4 * a.steps / dt
-    phases = {}
-    for lab in ("splinecoeff_xy", "advec2D", "field", "splinecoeff_vxvy", "advec4D"):
-        s = t_after[lab][0] - t_before[lab][0]
-        c = t_after[lab][1] - t_before[lab][1]
-        phases[lab] = round(1e3 * s / max(1, a.steps), 4)
-    adv_s = (t_after["advec4D"][0] - t_before["advec4D"][0]) / a.steps
-    variants = sim.kernel_variants()
-    adv_kernel = "k_advect4d_ys" if variants & 128 else "k_advect4d_blk"
-    adv_bytes = advect4d_bytes(n, b)
-    achieved = adv_bytes / adv_s / 1e9
-    valu = ADVECT4D_FLOP_PER_CELL * n ** 4 / adv_s / 1e12
-    d = sim.diags()
-    sim.close()
-    step_bytes = algorithmic_bytes(n, b)
+    modes = [a.precision] if a.precision else [33, 64, 32]
+    runs = {p: run_mode(pkg, n, p, a.steps, a.warmup) for p in modes}
+    head = runs[modes[0]]
     res = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": (value / A100_REF) if a.precision == 64 and n == 128 else None,
-        "dtype": {64: "f64", 32: "f32-storage, f64 arithmetic", 33: "f32-storage, f32 interpolation arithmetic"}[
-            a.precision],
+        # the only published figure is the A100 stdpar FP64 MainLoop of this deck
+        # (docs/vlp4d.md:148,168-171); no FP32 figure exists in the reference
+        "vs_baseline": head["value"] / A100_REF if n == 128 else None,
+        "dtype": head["dtype"],
         "data": "synthetic: SLD10 Landau-damping initial condition generated on the GPU",
         "config": {"workload": f"vlp4d spline step, SLD10 deck at {n}^4 (box 12.56637112, v in [-9,9], dt 0.01)",
-                   "grid": [n, n, n, n], "idcase": 10, "dt": 0.01, "storage_bits": 64 if a.precision == 64 else 32},
-        "roofline": {"bound": "hbm", "kernel": adv_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(adv_kernel, a.precision, n),
-                     "algorithmic_bytes_per_launch": adv_bytes, "avg_launch_ms": 1e3 * adv_s,
-                     "valu": {"achieved": valu, "peak": VALU_PEAK_TFS, "unit": "TFLOP/s",
-                              "frac": valu / VALU_PEAK_TFS, "flop_per_cell": ADVECT4D_FLOP_PER_CELL}},
-        "step_hbm_fraction": step_bytes / (dt / a.steps) / 1e9 / HBM_PEAK_GBS,
-        "step_algorithmic_bytes": step_bytes,
-        "phases_ms": phases,
-        "final_nrj": float(d["nrj"][-1]),
+                   "grid": [n, n, n, n], "idcase": 10, "dt": 0.01,
+                   "storage_bits": 64 if modes[0] == 64 else 32},
+        "roofline": head["roofline"],
+        "step_hbm_fraction": head["step_hbm_fraction"],
+        "step_algorithmic_bytes": head["step_algorithmic_bytes"],
+        "phases_ms": head["phases_ms"],
+        "final_nrj": head["final_nrj"],
     }
+    if 64 in runs and modes[0] != 64:
+        f64 = runs[64]
+        res["fp64"] = {k: f64[k] for k in ("value", "ms_per_step", "dtype", "roofline", "step_hbm_fraction",
+                                           "phases_ms", "final_nrj")}
+        res["fp64"]["vs_baseline"] = f64["value"] / A100_REF if n == 128 else None
+        res["speedup_vs_fp64"] = head["value"] / f64["value"]
+        w64 = [math.exp(2 * x) for x in f64["nrj"]]
+        for p in runs:
+            if p == 64:
+                continue
+            w = [math.exp(2 * x) for x in runs[p]["nrj"]]
+            key = "fp32_acc" if p == 33 else "fp32"
+            res.setdefault("field_energy_rel_err_vs_fp64", {})[key] = (
+                max(abs(u - v) for u, v in zip(w, w64)) / max(abs(v) for v in w64))
+        if 32 in runs:
+            r32 = runs[32]
+            res["fp32_f64arith"] = {"value": r32["value"], "ms_per_step": r32["ms_per_step"], "dtype": r32["dtype"],
+                                    "speedup_vs_fp64": r32["value"] / f64["value"]}
     if not a.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(res))

@@ -58,7 +58,7 @@ typedef struct {
   double minPhy[4];
   double maxPhy[4];
   double dx[4];       /* filled by the library                          */
-  int idcase;         /* 2, 5, 10, 20 (VMO/Init.cpp:129-146)             */
+  int idcase;         /* 2, 5, 10, 20 (VMO/Init.cpp:129-146), or 30, 31  */
   double dt;
   int nbiter;
   int ifreq;
@@ -66,6 +66,17 @@ typedef struct {
 } vlp_config;
 
 typedef struct vlp_ctx vlp_ctx;
+
+/* Initial conditions beyond the reference's (its dispatch, VMO/Init.cpp:
+ * 126-150, knows 2, 5, 10 and 20 and aborts on any other idcase); spline
+ * solver only.  Grid point i of dimension d is minPhy_d + i dx_d.
+ *   VLP_CASE_STRONG_LANDAU (30): (1 + 0.5 cos(0.5 x) + 0.5 cos(0.5 y))
+ *     exp(-(vx^2 + vy^2)/2) / (2 pi)
+ *   VLP_CASE_TWO_STREAM (31): (1 + 1e-3 cos(0.2 x)) exp(-vy^2/2)
+ *     [exp(-(vx - 2.4)^2/2) + exp(-(vx + 2.4)^2/2)] / (4 pi)
+ * The field solve treats them like any case without the rho -= 1 shift of
+ * idcases 10 and 20 (the k = 0 mode is dropped by the solve either way). */
+enum { VLP_CASE_STRONG_LANDAU = 30, VLP_CASE_TWO_STREAM = 31 };
 
 /* Phase identifiers for vlp_timer(): the labels of miniapps/vlp4d_mpi/Timer.hpp:37-66. */
 enum {
@@ -180,8 +191,10 @@ const char *vlp_timer_label(int which);
  * rows staged in LDS, bit 5: XCD-aware workgroup order, bit 6: advection
  * arithmetic in float, bit 7: y-streaming 4-D advection (replaces the
  * vx-blocked one), bits 8-11: its vx tile, bit 12: fused x/y spline + 2-D
- * advection kernel (replaces the separate x, y and advect2d kernels)).  No
- * reference counterpart: introspection for tests and reports. */
+ * advection kernel (replaces the separate x, y and advect2d kernels), bits
+ * 13 / 14: the vx / vy spline pass uses 128-position LDS tiles, four lanes
+ * per 128-point line).  No reference counterpart: introspection for tests
+ * and reports. */
 int vlp_kernel_variants(const vlp_ctx *ctx);
 
 /* Message of the last error on this context (or the last global error when
@@ -222,14 +235,48 @@ int vlp_adjoint_step(vlp_ctx *ctx, const double *g, double *grad, size_t n);
  * recomputed steps write the diagnostics a run would write).  No counterpart
  * in the reference. */
 int vlp_adjoint_steps(vlp_ctx *ctx, int nsteps, const double *g, double *grad, size_t n);
+/* Gradient of the field energy integrated over nsteps steps,
+ * J = dt * sum_{k=1..nsteps} W(f^{n+k}),  W(f) = sum_{x,y} (Ex^2 + Ey^2) dx dy
+ * of the field solved from f^{n+k} at the end of step k (the quantity whose
+ * log sqrt is the nrj diagnostic, VMO/Diags.cpp:16-48, recorded by
+ * VMO/Timestep.hpp:77-91), with respect to the context's current f^n
+ * (VLP_FP64), by the same checkpointed reverse sweep as vlp_adjoint_steps
+ * with the source dt dW/df added at every f^{n+k}; the forward sweep keeps
+ * every checkpoint_every-th state (<= 0: ceil(sqrt(nsteps))) and the reverse
+ * sweep recomputes the others.  *J receives J.
+ * vweight NULL: grad is dJ/df^n (n = Nx*Ny*Nvx*Nvy doubles, x fastest);
+ * else vweight holds Nvx*Nvy weights w(vx, vy) (vx fastest) and grad the
+ * moment sum_v w(v) dJ/df^n(x, y, v) (n = Nx*Ny) -- the gradient with
+ * respect to a density perturbation f^n += w(v) delta(x, y).  The context
+ * ends at its starting f and iteration.  VLP_E_ARG for idcases 2 and 6, whose
+ * field does not depend on f.  No counterpart in the reference. */
+int vlp_adjoint_field_energy(vlp_ctx *ctx, int nsteps, int checkpoint_every, const double *vweight, double *J,
+                             double *grad, size_t n);
+
+/* The field of a caller-held distribution function, without a context: the
+ * density (field_rho + field_reduce, VMO/Field.cpp:5-69, one rank), the
+ * Poisson solve (field_poisson, VMO/Field.cpp:71-137 / VMO/Efield.cpp:3-106),
+ * Diags::compute and computeL2norm (VMO/Diags.cpp:16-76) -- what the
+ * reference's onetimestep runs after each advection (VMO/Timestep.hpp:53-67,
+ * 77-91).  cfg gives the grid, box and idcase (dx is derived; dt, nbiter,
+ * ifreq, fxvx are ignored but must pass the vlp_create checks).  Device
+ * pointers: f holds Nx*Ny*Nvx*Nvy values, x fastest, double when f_bits is
+ * 64, float when 32; phi, ex, ey receive Nx*Ny doubles (phi: the reference's
+ * rho_ after the solve, i.e. the potential; rho itself for idcases 2 and 6);
+ * diag receives 5 doubles: nrj, nrjx, nrjy, mass, l2norm.  stream: a
+ * hipStream_t (NULL: the null stream).  Allocates its scratch (about
+ * 8 * Nx*Ny*Nvy bytes) per call and returns when the results are ready. */
+int vlp_field_solve(const vlp_config *cfg, const void *f, int f_bits, double *phi, double *ex, double *ey,
+                    double *diag, void *stream);
 
 /* Ensembles: independent contexts (any grid sizes and precisions, one
  * device) advanced together, two steps of every member per launch of one
  * captured HIP graph (their kernels overlap on the GPU); an odd remainder
  * runs eagerly.  Same results as vlp_step on each member.  No reference
  * counterpart (the reference runs one solve per process); for batched
- * studies of many small solves.  Destroy the ensemble before its members;
- * calls on the ensemble and on its members must not overlap. */
+ * studies of many small solves.  A context belongs to at most one ensemble
+ * at a time; vlp_destroy of a member returns VLP_E_ARG until the ensemble is
+ * destroyed.  Calls on the ensemble and on its members must not overlap. */
 typedef struct vlp_ensemble vlp_ensemble;
 int vlp_ensemble_create(vlp_ctx *const *members, int n, vlp_ensemble **out);
 /* Advance every member by nsteps (each member's iter + nsteps <= nbiter). */

@@ -6,7 +6,9 @@
  * per step, nrj.out / l2norm.out in the working directory and the timer table
  * of miniapps/vlp4d_mpi/Timer.hpp:70-75.  Used as the timed CPU baseline.
  *
- * Extra flag (not in the reference): "-n K" runs only K steps of the deck.
+ * Extra flags (not in the reference): "-n K" runs only K steps of the deck;
+ * "-T" prints "steptime <iter> <seconds>" after each step (wall time of that
+ * onetimestep + swap, the MainLoop body).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -47,10 +49,11 @@ static void print_config(const vo_config *c) { /* VMO/Init.cpp:92-124 */
 
 int main(int argc, char **argv) {
   const char *file = "data.dat";
-  int nsteps = -1;
+  int nsteps = -1, steptimes = 0;
   for (int i = 1; i < argc; i++) {
     if ((!strcmp(argv[i], "-f") || !strcmp(argv[i], "--file")) && i + 1 < argc) file = argv[++i];
     else if (!strcmp(argv[i], "-n") && i + 1 < argc) nsteps = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "-T")) steptimes = 1;
   }
   printf("reading input file %s\n", file);
   vo_config cfg;
@@ -75,7 +78,9 @@ int main(int argc, char **argv) {
   }
   while (vo_iter(st) < cfg.nbiter) {
     printf("iter %d\n", vo_iter(st));
+    const double t0 = vo_timer_seconds(st, VO_T_MAINLOOP);
     rc = vo_step(st);
+    if (steptimes && !rc) printf("steptime %d %.6f\n", vo_iter(st), vo_timer_seconds(st, VO_T_MAINLOOP) - t0);
     if (rc == VO_E_CFL) {
       fprintf(stderr, "Time step is too large, exiting\n");
       exit(0);

@@ -203,6 +203,34 @@ static void case_tsi20(const vo_config *c, double *fn) {  /* VMO/Init.cpp:247-27
     }
 }
 
+/* Extension cases with no reference counterpart (the reference's dispatch,
+ * VMO/Init.cpp:126-150, knows 2, 5, 10 and 20 only), same loop form as SLD10:
+ *   30  strong Landau damping: (1 + 0.5 cos(0.5 x) + 0.5 cos(0.5 y)) exp(-(vx^2 + vy^2)/2) / (2 pi)
+ *   31  two-stream: (1 + 1e-3 cos(0.2 x)) exp(-vy^2/2) [exp(-(vx-2.4)^2/2) + exp(-(vx+2.4)^2/2)] / (4 pi) */
+static void case_ext(const vo_config *c, double *fn) {
+#pragma omp parallel for collapse(2)
+  for (int ivy = 0; ivy <= c->nxmax[3] - 1; ivy++)
+    for (int ivx = 0; ivx <= c->nxmax[2] - 1; ivx++) {
+      const double vy = c->minPhy[3] + ivy * c->dx[3];
+      const double vx = c->minPhy[2] + ivx * c->dx[2];
+      for (int iy = 0; iy <= c->nxmax[1] - 1; iy++) {
+        const double y = c->minPhy[1] + iy * c->dx[1];
+        for (int ix = 0; ix <= c->nxmax[0] - 1; ix++) {
+          const double x = c->minPhy[0] + ix * c->dx[0];
+          double v;
+          if (c->idcase == 30) {
+            v = (1. / (2 * M_PI)) * exp(-0.5 * (vx * vx + vy * vy)) * (1 + 0.5 * cos(0.5 * x) + 0.5 * cos(0.5 * y));
+          } else {
+            const double a = vx - 2.4, b = vx + 2.4;
+            v = (1. / (4 * M_PI)) * (1 + 1e-3 * cos(0.2 * x)) * exp(-0.5 * (vy * vy)) *
+                (exp(-0.5 * (a * a)) + exp(-0.5 * (b * b)));
+          }
+          F4(c, fn, ix, iy, ivx, ivy) = v;
+        }
+      }
+    }
+}
+
 /* VMO/Init.cpp:126-150: fn is expected to be zero on entry (VMO/Init.cpp:370-371). */
 int vo_init_case(const vo_config *cfg, double *fn) {
   switch (cfg->idcase) {
@@ -210,6 +238,8 @@ int vo_init_case(const vo_config *cfg, double *fn) {
     case 5: case_cyl05(cfg, fn); break;
     case 10: case_sld10(cfg, fn); break;
     case 20: case_tsi20(cfg, fn); break;
+    case 30:
+    case 31: case_ext(cfg, fn); break;
     default: return VO_E_CASE;
   }
   return VO_OK;
@@ -939,7 +969,9 @@ int vo_create(const vo_config *cfg, vo_state **out) {
   *out = NULL;
   int rc = vo_config_check(cfg);
   if (rc) return rc;
-  if (cfg->idcase != 2 && cfg->idcase != 5 && cfg->idcase != 10 && cfg->idcase != 20) return VO_E_CASE;
+  if (cfg->idcase != 2 && cfg->idcase != 5 && cfg->idcase != 10 && cfg->idcase != 20 && cfg->idcase != 30 &&
+      cfg->idcase != 31)
+    return VO_E_CASE;
   vo_state *st = (vo_state *)calloc(1, sizeof(vo_state));
   if (!st) return VO_E_ARG;
   st->cfg = *cfg;

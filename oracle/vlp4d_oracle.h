@@ -68,7 +68,7 @@ size_t vo_halo_size(const vo_config *cfg);                   /* prod (N_d+6)    
 size_t vo_index(const vo_config *cfg, int ix, int iy, int ivx, int ivy);
 
 /* ---- phases (all operate on (N+6)^4 halo arrays) ----------------------- */
-int vo_init_case(const vo_config *cfg, double *fn);          /* VMO/Init.cpp:126-277 */
+int vo_init_case(const vo_config *cfg, double *fn);          /* VMO/Init.cpp:126-277, + extension cases 30, 31 */
 void vo_exchange_halo(const vo_config *cfg, double *fn);     /* VMO/Communication.cpp:541-570 */
 void vo_spline_xy(const vo_config *cfg, double *fn);         /* VMO/Spline.hpp:384-426 */
 void vo_spline_vxvy(const vo_config *cfg, double *fn);       /* VMO/Spline.hpp:429-469 */

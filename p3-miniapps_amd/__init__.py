@@ -25,6 +25,8 @@ OK, E_FILE, E_CASE, E_CFL, E_ARG, E_HIP, E_SIZE = range(7)
 FP64, FP32, FP32_ACC = 64, 32, 33
 SCHEME_SPLINE, SCHEME_LAGRANGE = 0, 1
 BUF_F, BUF_CXY, BUF_C4, BUF_RHO, BUF_EX, BUF_EY = range(6)
+# extension initial conditions (no reference counterpart), include/vlp4d.h
+CASE_STRONG_LANDAU, CASE_TWO_STREAM = 30, 31
 TIMER_LABELS = ["total", "MainLoop", "pack", "comm", "unpack", "advec2D", "advec4D", "field",
                 "all_reduce", "Fourier", "diag", "splinecoeff_xy", "splinecoeff_vxvy"]
 
@@ -36,6 +38,7 @@ EXPORTS = [
     "vlp_spline_vxvy", "vlp_advect4d_poisson", "vlp_timer", "vlp_timer_label", "vlp_last_error",
     "vlp_kernel_variants", "vlp_autotune", "vlp_tuned_tile", "vlp_ensemble_create", "vlp_ensemble_step",
     "vlp_ensemble_size", "vlp_ensemble_destroy", "vlp_adjoint_step", "vlp_adjoint_steps", "vlp_create_lagrange", "vlp_write_fxvx",
+    "vlp_field_solve", "vlp_adjoint_field_energy",
 ]
 
 
@@ -122,10 +125,36 @@ def load(path=LIB_PATH):
     lib.vlp_ensemble_destroy.argtypes = [vp]
     lib.vlp_adjoint_step.argtypes = [vp, dp, dp, ctypes.c_size_t]
     lib.vlp_adjoint_steps.argtypes = [vp, ctypes.c_int, dp, dp, ctypes.c_size_t]
+    lib.vlp_adjoint_field_energy.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp, dp, ctypes.c_size_t]
     lib.vlp_create_lagrange.argtypes = [P(VlpConfig), ctypes.c_int, P(vp)]
     lib.vlp_write_fxvx.argtypes = [vp, ctypes.c_char_p]
+    lib.vlp_field_solve.argtypes = [P(VlpConfig), vp, ctypes.c_int, vp, vp, vp, vp, vp]
     _lib = lib
     return lib
+
+
+def field_solve(cfg, f_ptr, f_bits, phi_ptr, ex_ptr, ey_ptr, diag_ptr, stream=None):
+    """vlp_field_solve on device pointers (integers): the density, Poisson
+    solve, field diagnostics and L2 norm of a device-resident f."""
+    lib = load()
+    rc = lib.vlp_field_solve(ctypes.byref(cfg), f_ptr, f_bits, phi_ptr, ex_ptr, ey_ptr, diag_ptr, stream)
+    if rc:
+        raise Vlp4dError(rc, lib.vlp_last_error(None).decode())
+
+
+def trig_interp_matrix(n, m):
+    """(m x n) periodic trigonometric interpolation from n to m equally spaced
+    samples of one period (n even): sample i at i/n, output j at j/m,
+    P[j, i] = (1/n) [1 + 2 sum_{k=1}^{n/2-1} cos(2 pi k s) + cos(pi n s)],
+    s = j/m - i/n -- the interpolant band-limited to |k| < n/2 with the
+    Nyquist mode split evenly between +-n/2.  It maps a density perturbation
+    given on a coarser periodic grid to the solver's grid (64 x 64 -> 96 x 96
+    as P @ delta @ P.T); its transpose carries the gradient back."""
+    if n % 2:
+        raise ValueError("n must be even")
+    s = np.arange(m)[:, None] / m - np.arange(n)[None, :] / n
+    k = np.arange(1, n // 2)
+    return (1.0 + 2.0 * np.cos(2 * np.pi * s[..., None] * k).sum(axis=-1) + np.cos(np.pi * n * s)) / n
 
 
 def _dptr(a):
@@ -173,8 +202,10 @@ class Vlp4d:
             raise Vlp4dError(rc, self.lib.vlp_last_error(self.h).decode())
 
     def close(self):
+        """Release the context.  Raises (and keeps the context) while it is a
+        member of a live Ensemble, whose captured graph uses its buffers."""
         if getattr(self, "h", None):
-            self.lib.vlp_destroy(self.h)
+            self._check(self.lib.vlp_destroy(self.h))
             self.h = None
 
     def __enter__(self):
@@ -243,6 +274,27 @@ class Vlp4d:
         self._check(self.lib.vlp_adjoint_steps(self.h, int(nsteps), _dptr(a), _dptr(out), a.size))
         return out.reshape(self.shape4[::-1])
 
+    def field_energy_gradient(self, nsteps, vweight=None, checkpoint_every=0):
+        """(J, dJ/df) for J = dt sum_{k=1..nsteps} sum_{x,y} |E(f^{n+k})|^2 dx dy
+        (FP64 contexts).  With vweight (array [Nvy][Nvx]) the gradient is the
+        moment sum_v w(v) dJ/df(x, y, v), shape [Ny][Nx]; otherwise the full
+        [Nvy][Nvx][Ny][Nx] array.  checkpoint_every: states kept by the
+        forward sweep (0: ceil(sqrt(nsteps))).  The context ends where it
+        started."""
+        J = ctypes.c_double()
+        nx, ny = self.shape4[0], self.shape4[1]
+        if vweight is None:
+            out = np.empty(self.shape4[::-1])
+            self._check(self.lib.vlp_adjoint_field_energy(self.h, int(nsteps), int(checkpoint_every), None, ctypes.byref(J), _dptr(out),
+                                                          out.size))
+        else:
+            w = np.ascontiguousarray(vweight, dtype=np.float64)
+            assert w.shape == (self.shape4[3], self.shape4[2])
+            out = np.empty((ny, nx))
+            self._check(self.lib.vlp_adjoint_field_energy(self.h, int(nsteps), int(checkpoint_every), _dptr(w), ctypes.byref(J), _dptr(out),
+                                                          out.size))
+        return J.value, out
+
     def buffer(self, which):
         n = self.lib.vlp_buffer_size(self.h, which)
         out = np.empty(n, dtype=np.float64)
@@ -297,7 +349,8 @@ class Vlp4d:
 class Ensemble:
     """Independent simulations (Vlp4d contexts on one device, any grids and
     precisions) advanced together through one captured HIP graph per two
-    steps.  Close the ensemble before its members."""
+    steps.  Close the ensemble before its members (closing a member of a
+    live ensemble raises)."""
 
     def __init__(self, members):
         self.lib = load()

@@ -187,6 +187,45 @@ __global__ __launch_bounds__(256) void k_adj_advect2d(const double *__restrict__
     for (int kx = 0; kx < 4; kx++) atomicAdd(pl + (long)(jy0 + 1 + ky) * E0 + (jx0 + 1 + kx), lv * ex[kx] * ey[ky]);
 }
 
+// Field-energy source: W(f) = dx dy sum (Ex^2 + Ey^2) of the field solved
+// from f, so dW/dE = 2 dx dy E.  lE = scale * (Ex, Ey) pairs.
+__global__ __launch_bounds__(256) void k_scale_pairs(const double *__restrict__ exy, double scale, long n,
+                                                     double *__restrict__ lE) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) lE[t] = scale * exy[t];
+}
+
+// g(x, y, v) += scale * lrho(x, y) for every (vx, vy): the density's adjoint
+// (rho = dvx dvy sum_v f).
+__global__ __launch_bounds__(256) void k_add_density_adj(const double *__restrict__ lrho, double scale, long nxy,
+                                                         long ncell, double *__restrict__ gbuf) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ncell) gbuf[t] += scale * lrho[t % nxy];
+}
+
+// out(x, y) = sum_{vx, vy} w(vx, vy) in(x, y, vx, vy), summed in a fixed order.
+__global__ __launch_bounds__(256) void k_vmoment(const double *__restrict__ in, const double *__restrict__ w, long nxy,
+                                                 long nv, double *__restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nxy) return;
+  double s = 0.;
+  for (long v = 0; v < nv; v++) s += w[v] * in[t + nxy * v];
+  out[t] = s;
+}
+
+void launch_scale_pairs(const double *exy, double scale, long n, double *lE, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, exy, scale, n, lE);
+}
+
+void launch_add_density_adj(const double *lrho, double scale, long nxy, long ncell, double *g, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_density_adj, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, lrho, scale, nxy,
+                     ncell, g);
+}
+
+void launch_vmoment(const double *in, const double *w, long nxy, long nv, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_vmoment, dim3((unsigned)((nxy + 255) / 256)), dim3(256), 0, s, in, w, nxy, nv, out);
+}
+
 void launch_adj_advect4d(const Geom &g, const double *c4, const double *ex, const double *ey, double dt,
                          const double *gin, double *lc4, double *lE, bool field_term, hipStream_t s) {
   const long n = (long)g.n[0] * g.n[1] * g.n[2] * g.n[3];

@@ -436,6 +436,21 @@ template <typename T> __global__ __launch_bounds__(256) void k_init_case(T *__re
             (exp(-.5 * (a1 * a1 + a2 * a2)) + exp(-.5 * (a3 * a3 + a2 * a2)));
       break;
     }
+    case 30:    // extension: strong Landau damping (no reference counterpart)
+    case 31: {  // extension: two-stream (no reference counterpart)
+      const double vy = g.minPhy[3] + ivy * g.dx[3];
+      const double vx = g.minPhy[2] + ivx * g.dx[2];
+      const double y = g.minPhy[1] + iy * g.dx[1];
+      const double x = g.minPhy[0] + ix * g.dx[0];
+      if (g.idcase == 30) {
+        val = (1. / (2 * M_PI)) * exp(-0.5 * (vx * vx + vy * vy)) * (1 + 0.5 * cos(0.5 * x) + 0.5 * cos(0.5 * y));
+      } else {
+        const double a = vx - 2.4, b = vx + 2.4;
+        val = (1. / (4 * M_PI)) * (1 + 1e-3 * cos(0.2 * x)) * exp(-0.5 * (vy * vy)) *
+              (exp(-0.5 * (a * a)) + exp(-0.5 * (b * b)));
+      }
+      break;
+    }
     default:
       return;
   }

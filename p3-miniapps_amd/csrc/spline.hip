@@ -463,6 +463,11 @@ template <typename T> void launch_spline_line(const T *in, T *out, const LineMap
 }
 #undef VLP_LINE_CASE
 
+int spline_tile_width(const LineMap &m, size_t tsz) {
+  int P, XT;
+  return tile_chunk(m, tsz, P, XT) ? XT : 0;
+}
+
 int spline_variant_bits(const Geom &g) {
   int P, logP, C;
   x_split(g.n[0], P, logP, C);

@@ -57,6 +57,7 @@ struct vlp_ctx {
   int tcalls[VLP_T_COUNT] = {};
   hipEvent_t ev[8] = {};
   int ys_tvx = 0, ys_nb = 0;  // 4-D advection tile chosen by vlp_autotune (0: preference order)
+  vlp_ensemble *owner = nullptr;  // the ensemble whose captured graph holds this context's buffers
   // adjoint of one step (allocated on first use)
   double *adj_c4s = nullptr, *adj_blk = nullptr, *adj_tmp = nullptr, *adj_cxy = nullptr;
   double *adj_E = nullptr, *adj_rho = nullptr, *adj_k = nullptr, *adj_g = nullptr, *adj_grad = nullptr;
@@ -105,7 +106,8 @@ int check_config(const vlp_config *c) {
     if (c->nxmax[i] + 5 > 256) return set_err(nullptr, VLP_E_SIZE, "dimension larger than 251 (spline line buffer)");
   }
   if (c->nbiter < 0 || c->ifreq <= 0) return set_err(nullptr, VLP_E_ARG, "nbiter < 0 or ifreq <= 0");
-  if (c->idcase != 2 && c->idcase != 5 && c->idcase != 10 && c->idcase != 20)
+  if (c->idcase != 2 && c->idcase != 5 && c->idcase != 10 && c->idcase != 20 && c->idcase != VLP_CASE_STRONG_LANDAU &&
+      c->idcase != VLP_CASE_TWO_STREAM)
     return set_err(nullptr, VLP_E_CASE, "Unknown test case !");
   if (!(c->dt > 0) || !std::isfinite(c->dt)) return set_err(nullptr, VLP_E_ARG, "dt must be positive");
   for (int i = 0; i < 4; i++)
@@ -183,27 +185,36 @@ template <typename T> int run_spline_xy(vlp_ctx *c) {
   return VLP_OK;
 }
 
-template <typename T> int run_spline_vxvy(vlp_ctx *c) {
-  const Geom &g = c->g;
+// vx-pass: cxy (E0*E1, N2, N3) -> tv (E0*E1, E2, N3); lanes along (x, y)
+vlp::LineMap vx_map(const Geom &g) {
   const long N2 = g.n[2], N3 = g.n[3];
-  const long E0 = g.n[0] + 4, E1 = g.n[1] + 4, E2 = N2 + 4;
-  const long F0 = g.n[0] + 6, F1 = g.n[1] + 6, F2 = N2 + 6;
-  const long P = E0 * E1;
-  // vx-pass: cxy (E0*E1, N2, N3) -> tv (E0*E1, E2, N3); lanes along (x, y)
+  const long P = (long)(g.n[0] + 4) * (g.n[1] + 4), E2 = N2 + 4;
   vlp::LineMap m{};
   m.N = (int)N2;
   m.X0 = P; m.X1 = N3; m.A = 1;
   m.is0 = 1; m.is1 = P * N2; m.ios = 0; m.ils = P;
   m.os0 = 1; m.os1 = P * E2; m.oos = 0; m.ols = P;
-  vlp::launch_spline_line<T>((const T *)c->cxy, (T *)c->tv, m, c->stream);
-  // vy-pass: tv (E0, E1, E2, N3) -> c4 (F0, F1, F2, F3) at positions -2..N+1
+  return m;
+}
+
+// vy-pass: tv (E0, E1, E2, N3) -> c4 (F0, F1, F2, F3) at positions -2..N+1
+vlp::LineMap vy_map(const Geom &g) {
+  const long E0 = g.n[0] + 4, E1 = g.n[1] + 4, E2 = g.n[2] + 4;
+  const long F0 = g.n[0] + 6, F1 = g.n[1] + 6, F2 = g.n[2] + 6;
   vlp::LineMap q{};
-  q.N = (int)N3;
+  q.N = g.n[3];
   q.X0 = E0; q.X1 = E1; q.A = E2;
   q.is0 = 1; q.is1 = E0; q.ios = E0 * E1; q.ils = E0 * E1 * E2;
   q.os0 = 1; q.os1 = F0; q.oos = F0 * F1; q.ols = F0 * F1 * F2;
+  return q;
+}
+
+template <typename T> int run_spline_vxvy(vlp_ctx *c) {
+  const Geom &g = c->g;
+  const long F0 = g.n[0] + 6, F1 = g.n[1] + 6, F2 = g.n[2] + 6;
+  vlp::launch_spline_line<T>((const T *)c->cxy, (T *)c->tv, vx_map(g), c->stream);
   T *c4 = (T *)c->c4 + 1 + F0 * (1 + F1 * (1 + F2 * 1));
-  vlp::launch_spline_line<T>((const T *)c->tv, c4, q, c->stream);
+  vlp::launch_spline_line<T>((const T *)c->tv, c4, vy_map(g), c->stream);
   return VLP_OK;
 }
 
@@ -438,6 +449,14 @@ int refresh_from_f(vlp_ctx *c) {
 
 }  // namespace
 
+namespace vlp {
+int set_error(int code, const char *msg) { return set_err(nullptr, code, msg); }
+int prepare_config(vlp_config *cfg) {
+  derive_dx(cfg);
+  return check_config(cfg);
+}
+}  // namespace vlp
+
 extern "C" {
 
 const char *vlp_version(void) { return "vlp4d-mi355x 0.1 (gfx950)"; }
@@ -498,6 +517,8 @@ static int create_ctx(const vlp_config *cfg_in, int precision, int device, int s
   derive_dx(&cfg);
   int rc = check_config(&cfg);
   if (rc) return rc;
+  if (lag && cfg.idcase != 2 && cfg.idcase != 5 && cfg.idcase != 10 && cfg.idcase != 20)
+    return set_err(nullptr, VLP_E_CASE, "Unknown test case !");  // miniapps/vlp4d/openmp/init.cpp:124-142
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return set_err(nullptr, VLP_E_HIP, "no HIP device available");
@@ -600,6 +621,8 @@ static int create_ctx(const vlp_config *cfg_in, int precision, int device, int s
 
 int vlp_destroy(vlp_ctx *c) {
   if (!c) return VLP_OK;
+  if (c->owner)  // its buffers are baked into the ensemble's graph
+    return set_err(c, VLP_E_ARG, "context is a member of a live ensemble; destroy the ensemble first");
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_all(c);
@@ -843,6 +866,8 @@ int vlp_kernel_variants(const vlp_ctx *c) {
   if (c->scheme == VLP_SCHEME_LAGRANGE) return vlp::lag_xy_chunks(c->g) > 0 ? 1 : 0;  // bit 0: fused x/y pass
   int bits = vlp::kernel_variants(c->g, c->precision != VLP_FP64, c->precision == VLP_FP32_ACC);
   if ((bits & 128) && c->ys_tvx) bits = (bits & ~(15 << 8)) | (c->ys_tvx << 8);
+  if (vlp::spline_tile_width(vx_map(c->g), c->esz) == 128) bits |= 1 << 13;
+  if (vlp::spline_tile_width(vy_map(c->g), c->esz) == 128) bits |= 1 << 14;
   return bits;
 }
 
@@ -1033,21 +1058,37 @@ int vlp_adjoint_step(vlp_ctx *c, const double *g_host, double *grad_host, size_t
   return sync_and_check(c);  // the recomputed phases flag a foot out of the box
 }
 
-// Adjoint of nsteps steps from the context's f: grad = (d f^{n+K}/d f^n)^T g
-// by two-level checkpointing -- the forward sweep keeps every C-th state (C =
-// ceil(sqrt(K))); the reverse sweep recomputes each segment's states from its
-// checkpoint and applies the one-step adjoint backwards through them.  Device
-// memory: about 2 sqrt(K) copies of f.  The context ends at its starting f and
-// iteration; the diagnostics of the recomputed steps are written as a run
-// would write them.
-int vlp_adjoint_steps(vlp_ctx *c, int nsteps, const double *g_host, double *grad_host, size_t n) {
-  int rc = adjoint_args(c, g_host, grad_host, n);
+// Adds the field-energy source of the context's current f to adj_g:
+// W(f) = dx dy sum (Ex^2 + Ey^2) of the field solved from f (the density,
+// Poisson solve and diagnostics of VMO/Timestep.hpp:77-91), scaled by `w`,
+// has dW/df(x, y, v) = dvx dvy (K^T (2 dx dy E))(x, y) for every v.
+static int add_field_energy_source(vlp_ctx *c, double w) {
+  const Geom &g = c->g;
+  const long nxy = (long)g.n[0] * g.n[1];
+  hipStream_t s = c->stream;
+  run_moments<double>(c);
+  int rc = field(c, c->iter, false, false);
   if (rc) return rc;
-  if (nsteps < 1 || c->iter + nsteps > c->cfg.nbiter)
-    return set_err(c, VLP_E_ARG, "nsteps must be at least 1 and within nbiter");
+  vlp::launch_scale_pairs(c->fb.exy, w * 2. * g.dx[0] * g.dx[1], 2 * nxy, c->adj_E, s);
+  vlp::launch_poisson_T(c->adj_E, c->adj_k, c->adj_k + nxy, c->adj_rho, g.n[0], g.n[1], s);
+  vlp::launch_add_density_adj(c->adj_rho, g.dx[2] * g.dx[3], nxy, (long)c->n_f, c->adj_g, s);
+  return check_launch(c, "field-energy source");
+}
+
+// Reverse sweep over nsteps steps from the context's f by two-level
+// checkpointing -- the forward sweep keeps every C-th state (C = `every`, or
+// ceil(sqrt(K)) when every <= 0); the reverse sweep recomputes each segment's states from its
+// checkpoint and applies the one-step adjoint backwards through them.  Device
+// memory: K/C + C copies of f.  The terminal cotangent is g_host (or
+// zero when null); with `energy`, the source dt W(f^{n+k}) of every step k =
+// 1..K is added as the sweep passes f^{n+k}, and *J receives
+// dt sum_k W(f^{n+k}) from the forward sweep's diagnostics.  The result,
+// d/df^n of <g, f^{n+K}> (+ J), is left in adj_g.  The context ends at its
+// starting f and iteration, also after a failure part-way through.
+static int adjoint_sweep(vlp_ctx *c, int K, int every, const double *g_host, bool energy, double *J) {
   hipStream_t s = c->stream;
   const size_t fb = c->n_f * sizeof(double);
-  const int K = nsteps, C = (int)std::ceil(std::sqrt((double)K)), nck = (K + C - 1) / C;
+  const int C = every > 0 ? std::min(every, K) : (int)std::ceil(std::sqrt((double)K)), nck = (K + C - 1) / C;
   std::vector<double *> ck(nck, nullptr), seg(C, nullptr);
   auto release = [&]() {
     for (double *p : ck)
@@ -1062,40 +1103,124 @@ int vlp_adjoint_steps(vlp_ctx *c, int nsteps, const double *g_host, double *grad
         return set_err(c, VLP_E_HIP, "hipMalloc failed (adjoint checkpoints)");
       }
   const int iter0 = c->iter;
+  const double dt = c->cfg.dt;
   auto restore = [&](const double *src, int it) -> int {
     HIP_TRY(c, hipMemcpyAsync(c->f, src, fb, hipMemcpyDeviceToDevice, s));
     c->iter = it;
     return VLP_OK;
   };
+  c->err.clear();
+  int rc = VLP_OK;
   // forward sweep: checkpoints at 0, C, 2C, ...
   for (int k = 0; k < K && !rc; k++) {
     if (k % C == 0) {
-      if (hipMemcpyAsync(ck[k / C], c->f, fb, hipMemcpyDeviceToDevice, s) != hipSuccess) rc = VLP_E_HIP;
+      if (hipMemcpyAsync(ck[k / C], c->f, fb, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        rc = set_err(c, VLP_E_HIP, "adjoint: checkpoint copy failed");
     }
     if (!rc) rc = step_once(c);
   }
-  if (!rc && hipMemcpyAsync(c->adj_g, g_host, fb, hipMemcpyHostToDevice, s) != hipSuccess) rc = VLP_E_HIP;
+  if (!rc && energy && J) {  // W_k = 2 (nrjx_k^2 + nrjy_k^2) from the recorded diagnostics
+    const int nb = c->cfg.nbiter + 1;
+    std::vector<double> h(5 * (size_t)nb);
+    if (hipMemcpyAsync(h.data(), c->fb.diag, h.size() * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      rc = set_err(c, VLP_E_HIP, "adjoint: diagnostics copy failed");
+    double sum = 0.;
+    for (int k = 1; k <= K && !rc; k++) {
+      const double a = h[nb + iter0 + k], b = h[2 * (size_t)nb + iter0 + k];
+      sum += 2. * (a * a + b * b);
+    }
+    *J = dt * sum;
+  }
+  if (!rc) {
+    if (g_host) {
+      if (hipMemcpyAsync(c->adj_g, g_host, fb, hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = set_err(c, VLP_E_HIP, "adjoint: copy of the cotangent failed");
+    } else if (hipMemsetAsync(c->adj_g, 0, fb, s) != hipSuccess) {
+      rc = set_err(c, VLP_E_HIP, "adjoint: clearing the cotangent failed");
+    }
+  }
+  if (!rc && energy) rc = add_field_energy_source(c, dt);  // f is f^{n+K}
   // reverse sweep, segment by segment
   for (int sg = nck - 1; sg >= 0 && !rc; sg--) {
     const int k0 = sg * C, len = std::min(C, K - k0);
     rc = restore(ck[sg], iter0 + k0);
     for (int j = 0; j < len && !rc; j++) {  // the segment's states
-      if (hipMemcpyAsync(seg[j], c->f, fb, hipMemcpyDeviceToDevice, s) != hipSuccess) rc = VLP_E_HIP;
+      if (hipMemcpyAsync(seg[j], c->f, fb, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        rc = set_err(c, VLP_E_HIP, "adjoint: segment copy failed");
       if (!rc && j + 1 < len) rc = step_once(c);
     }
     for (int j = len - 1; j >= 0 && !rc; j--) {
       rc = restore(seg[j], iter0 + k0 + j);
       if (!rc) rc = adjoint_core(c);
       if (!rc) std::swap(c->adj_g, c->adj_grad);  // the result is the next cotangent
+      if (!rc && energy && k0 + j >= 1) rc = add_field_energy_source(c, dt);  // f is f^{n+k0+j}
     }
   }
-  if (!rc) rc = restore(ck[0], iter0);
-  if (!rc && hipMemcpyAsync(grad_host, c->adj_g, fb, hipMemcpyDeviceToHost, s) != hipSuccess) rc = VLP_E_HIP;
+  // back to the starting state, also after a failure part-way through
+  const std::string err = c->err;
+  const bool restored = hipStreamSynchronize(s) == hipSuccess &&
+                        hipMemcpyAsync(c->f, ck[0], fb, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+                        hipStreamSynchronize(s) == hipSuccess;
+  c->iter = iter0;
+  if (!restored) {
+    c->broken = true;
+    if (!rc) rc = set_err(c, VLP_E_HIP, "adjoint: restoring the starting state failed");
+  } else if (rc) {
+    // f and iter are the starting ones again; clear the step's error flag
+    if (hipMemsetAsync(c->d_err, 0, sizeof(int), s) == hipSuccess && hipStreamSynchronize(s) == hipSuccess)
+      c->broken = false;
+    c->err = err;
+  }
   if (!rc) rc = sync_and_check(c);
-  else (void)hipStreamSynchronize(s);
   release();
   if (rc == VLP_E_HIP && c->err.empty()) return set_err(c, VLP_E_HIP, "adjoint sweep failed");
   return rc;
+}
+
+int vlp_adjoint_steps(vlp_ctx *c, int nsteps, const double *g_host, double *grad_host, size_t n) {
+  int rc = adjoint_args(c, g_host, grad_host, n);
+  if (rc) return rc;
+  if (nsteps < 1 || c->iter + nsteps > c->cfg.nbiter)
+    return set_err(c, VLP_E_ARG, "nsteps must be at least 1 and within nbiter");
+  rc = adjoint_sweep(c, nsteps, 0, g_host, false, nullptr);
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(grad_host, c->adj_g, c->n_f * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return VLP_OK;
+}
+
+int vlp_adjoint_field_energy(vlp_ctx *c, int nsteps, int checkpoint_every, const double *vweight, double *J,
+                             double *grad_host, size_t n) {
+  int rc = spline_only(c);
+  if (rc) return rc;
+  if (c->precision != VLP_FP64) return set_err(c, VLP_E_ARG, "the adjoint is implemented for VLP_FP64 contexts");
+  const size_t nxy = (size_t)c->cfg.nxmax[0] * c->cfg.nxmax[1];
+  const size_t nv = (size_t)c->cfg.nxmax[2] * c->cfg.nxmax[3];
+  if (!J || !grad_host || n != (vweight ? nxy : c->n_f)) return set_err(c, VLP_E_SIZE, "gradient buffer size mismatch");
+  if (nsteps < 1 || c->iter + nsteps > c->cfg.nbiter)
+    return set_err(c, VLP_E_ARG, "nsteps must be at least 1 and within nbiter");
+  rc = adjoint_prepare(c);
+  if (rc) return rc;
+  if (!c->adj_k) return set_err(c, VLP_E_ARG, "the field energy of idcases 2 and 6 does not depend on f");
+  rc = adjoint_sweep(c, nsteps, checkpoint_every, nullptr, true, J);
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  if (!vweight) {
+    HIP_TRY(c, hipMemcpyAsync(grad_host, c->adj_g, c->n_f * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return VLP_OK;
+  }
+  // the velocity moment on the device; the weights and the moment sit in
+  // the adjoint's gradient buffer, free after the sweep (N^4 >= nv + nxy)
+  double *w = c->adj_grad, *out = c->adj_grad + nv;
+  HIP_TRY(c, hipMemcpyAsync(w, vweight, nv * sizeof(double), hipMemcpyHostToDevice, s));
+  vlp::launch_vmoment(c->adj_g, w, (long)nxy, (long)nv, out, s);
+  rc = check_launch(c, "velocity moment");
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(grad_host, out, nxy * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return VLP_OK;
 }
 
 int vlp_tuned_tile(const vlp_ctx *c, int *vx_tile, int *plane_buffers) {
@@ -1128,6 +1253,7 @@ struct vlp_ensemble {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<void *> cap_f;  // each member's f when the graph was captured
+  std::vector<int> cap_tile;  // and its 4-D advection tile (ys_tvx * 16 + ys_nb)
   double seconds = 0.;
   long steps = 0;
 };
@@ -1176,7 +1302,11 @@ int ensemble_capture(vlp_ensemble *e) {
   if (hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0) != hipSuccess)
     return set_err(nullptr, VLP_E_HIP, "hipGraphInstantiate failed");
   e->cap_f.resize(e->m.size());
-  for (size_t i = 0; i < e->m.size(); i++) e->cap_f[i] = e->m[i]->f;
+  e->cap_tile.resize(e->m.size());
+  for (size_t i = 0; i < e->m.size(); i++) {
+    e->cap_f[i] = e->m[i]->f;
+    e->cap_tile[i] = e->m[i]->ys_tvx * 16 + e->m[i]->ys_nb;
+  }
   return VLP_OK;
 }
 
@@ -1194,6 +1324,7 @@ int vlp_ensemble_create(vlp_ctx *const *members, int n, vlp_ensemble **out) {
     if (members[i]->device != members[0]->device) return set_err(nullptr, VLP_E_ARG, "members on different devices");
     if (members[i]->scheme != VLP_SCHEME_SPLINE)
       return set_err(nullptr, VLP_E_ARG, "ensembles take spline-solver contexts only");
+    if (members[i]->owner) return set_err(nullptr, VLP_E_ARG, "a member already belongs to an ensemble");
     for (int j = 0; j < i; j++)
       if (members[j] == members[i]) return set_err(nullptr, VLP_E_ARG, "a context appears twice");
   }
@@ -1211,6 +1342,7 @@ int vlp_ensemble_create(vlp_ctx *const *members, int n, vlp_ensemble **out) {
     delete e;
     return set_err(nullptr, VLP_E_HIP, "ensemble set-up failed");
   }
+  for (auto *c : e->m) c->owner = e;
   *out = e;
   return VLP_OK;
 }
@@ -1228,8 +1360,10 @@ int vlp_ensemble_step(vlp_ensemble *e, int nsteps) {
   if (pairs > 0) {
     for (auto *c : e->m)
       if (hipStreamSynchronize(c->stream) != hipSuccess) return set_err(nullptr, VLP_E_HIP, "member stream");
+    // the graph holds each member's buffers and tile as captured
     bool fresh = e->exec != nullptr;
-    for (size_t i = 0; i < e->m.size() && fresh; i++) fresh = e->cap_f[i] == e->m[i]->f;
+    for (size_t i = 0; i < e->m.size() && fresh; i++)
+      fresh = e->cap_f[i] == e->m[i]->f && e->cap_tile[i] == e->m[i]->ys_tvx * 16 + e->m[i]->ys_nb;
     if (!fresh) {
       const int rc = ensemble_capture(e);
       if (rc) return rc;
@@ -1240,12 +1374,23 @@ int vlp_ensemble_step(vlp_ensemble *e, int nsteps) {
       return set_err(nullptr, VLP_E_HIP, "iteration counters");
     for (int k = 0; k < pairs; k++)
       if (hipGraphLaunch(e->exec, e->root) != hipSuccess) return set_err(nullptr, VLP_E_HIP, "hipGraphLaunch failed");
-    if (hipStreamSynchronize(e->root) != hipSuccess) return set_err(nullptr, VLP_E_HIP, "ensemble replay failed");
-    for (auto *c : e->m) {
-      c->iter += 2 * pairs;
-      const int rc = sync_and_check(c);
-      if (rc) return set_err(nullptr, rc, c->err);
+    if (hipStreamSynchronize(e->root) != hipSuccess) {
+      for (auto *c : e->m) c->broken = true;  // the members' states are unknown
+      return set_err(nullptr, VLP_E_HIP, "ensemble replay failed");
     }
+    // every member has advanced: count the steps on all of them, then read
+    // every member's error flag and report the first failure
+    for (auto *c : e->m) c->iter += 2 * pairs;
+    int first = VLP_OK;
+    std::string msg;
+    for (auto *c : e->m) {
+      const int rc = sync_and_check(c);
+      if (rc && !first) {
+        first = rc;
+        msg = c->err;
+      }
+    }
+    if (first) return set_err(nullptr, first, msg);
   }
   if (nsteps % 2)
     for (auto *c : e->m) {
@@ -1269,6 +1414,7 @@ int vlp_ensemble_destroy(vlp_ensemble *e) {
   if (!e) return VLP_OK;
   (void)hipSetDevice(e->device);
   if (e->root) (void)hipStreamSynchronize(e->root);
+  for (auto *c : e->m) c->owner = nullptr;
   ensemble_free(e);
   delete e;
   return VLP_OK;

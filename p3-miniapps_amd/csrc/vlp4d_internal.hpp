@@ -17,7 +17,15 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "../../include/vlp4d.h"
+
 namespace vlp {
+
+// Host helpers of vlp4d_api.cpp: record an error for vlp_last_error(NULL)
+// (returns code); fill dx from the box and check the configuration the way
+// vlp_create does (VLP_OK or an error code, already recorded).
+int set_error(int code, const char *msg);
+int prepare_config(vlp_config *cfg);
 
 // Everything a kernel needs from the configuration (one rank).
 struct Geom {
@@ -45,6 +53,9 @@ struct LineMap {
 template <typename T> void launch_init_case(const Geom &g, T *f, hipStream_t s);
 template <typename T> void launch_spline_x(const Geom &g, const T *f, T *tx, hipStream_t s);
 template <typename T> void launch_spline_line(const T *in, T *out, const LineMap &m, hipStream_t s);
+// Tile width (64 or 128 positions) of the LDS-tiled kernel launch_spline_line
+// picks for this line family and storage size, 0 when it takes a line kernel.
+int spline_tile_width(const LineMap &m, size_t tsz);
 template <typename T>
 void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, bool arith32, double *rho_part, int *err,
                         hipStream_t s);
@@ -87,7 +98,8 @@ long l2_partials(const Geom &g);
 // E rows staged in LDS, bit 5: XCD-aware workgroup order, bit 6: float
 // advection arithmetic, bit 7: the y-streaming 4-D advection replaces the
 // vx-blocked one, bits 8-11: its vx tile, bit 12: the fused xy spline + 2-D
-// advection kernel replaces the x, y and advect2d kernels.
+// advection kernel replaces the x, y and advect2d kernels, bits 13 / 14: the
+// vx / vy spline pass runs 128-position tiles (four lanes per 128-point line).
 int kernel_variants(const Geom &g, bool fp32_storage, bool arith32);
 
 // Adjoint of one step (adjoint.hip), FP64.
@@ -114,6 +126,11 @@ void launch_c4_block(const Geom &g, const double *c4s, double *blk, hipStream_t 
 void launch_poisson_T(const double *lE, const double *kx, const double *ky, double *lrho, int N0, int N1,
                       hipStream_t s);
 void launch_adj_advect2d(const double *lrho, const Geom &g, double dth, double *lcxy, hipStream_t s);
+// Field-energy adjoint helpers: lE = scale * exy (n doubles); g(x, y, v) +=
+// scale * lrho(x, y) over ncell cells; out(x, y) = sum_v w(v) in(x, y, v).
+void launch_scale_pairs(const double *exy, double scale, long n, double *lE, hipStream_t s);
+void launch_add_density_adj(const double *lrho, double scale, long nxy, long ncell, double *g, hipStream_t s);
+void launch_vmoment(const double *in, const double *w, long nxy, long nv, double *out, hipStream_t s);
 
 // Field solve (FP64): rho from the per-vy partial sums, Poisson by DFT with
 // the reference's FFT conventions, field diagnostics; optional L2 norm.
@@ -137,5 +154,13 @@ constexpr int RHO_SLICES = 16;
 // diagnostics then go to index *iter_dev + 1, which is stored back.
 void launch_field(const Geom &g, const FieldBufs &b, const double *rho_part, long rho_outer, const double *l2_part,
                   long n_l2, int iter, bool record_diag, hipStream_t s, int *iter_dev = nullptr);
+
+// Field solve of a caller-held f (field_solve.hip): field_solve_geom checks
+// the arguments, fills g and the scratch size in doubles; field_solve_enqueue
+// launches the moments, twiddle and field kernels on s with that scratch.
+int field_solve_geom(const vlp_config *cfg, const void *f, int f_bits, double *phi, double *ex, double *ey,
+                     double *diag, Geom &g, size_t &scratch_doubles);
+int field_solve_enqueue(const Geom &g, const void *f, int f_bits, double *phi, double *ex, double *ey,
+                        double *diag, double *scratch, hipStream_t s);
 
 }  // namespace vlp

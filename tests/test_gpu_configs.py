@@ -1,11 +1,25 @@
 """The workload configurations at their stated sizes, through the C-ABI.
 
-* 32^4 strong-Landau box (x, y in [0, 4 pi), v in [-6, 6]): the stated
-  dt = 0.1 gives dt*V/dx = 1.53, for which the reference prints "Time step is
-  too large" and exits (VMO/Advection.hpp:456-460); the library reports
-  VLP_E_CFL on the first step.  Its parity twin -- the reference's SLD10
-  initial condition (VMO/Init.cpp:226-245) on that grid with dt = 0.05 --
-  runs 100 steps against the oracle at the 1e-12 bar.
+* 32^4 strong-Landau box (x, y in [0, 4 pi), v in [-6, 6]) with its own
+  initial condition, f0 = (1 + 0.5 cos(x/2) + 0.5 cos(y/2)) e^{-|v|^2/2} / 2 pi
+  (extension idcase 30): the stated dt = 0.1 gives dt*V/dx = 1.53, for which
+  the reference prints "Time step is too large" and exits
+  (VMO/Advection.hpp:456-460); the library reports VLP_E_CFL on the first
+  step.  At dt = 0.05 (ratio 0.76) it runs 100 steps against the oracle at
+  the 1e-12 bar, and so does the reference's SLD10 initial condition
+  (VMO/Init.cpp:226-245) on that grid.
+* 64^4 two-stream box with its own initial condition, f0 = (1 + 1e-3
+  cos(0.2 x)) e^{-vy^2/2} [e^{-(vx-2.4)^2/2} + e^{-(vx+2.4)^2/2}] / 4 pi
+  (extension idcase 31), dt = 0.05, all 400 steps against the oracle's run
+  stored in tests/golden/two_stream_64.npz (made by make_two_stream_64.py):
+  f (every 8th point per axis) at steps 100, 200, 300, 400 and the nrjx and
+  L2 series at 1e-12.  The field of this state is the 1e-3 perturbation of
+  a uniform density, so its norm carries the rounding of rho = sum_v f
+  relative to that perturbation: ||E|| = exp(nrj) is held at 1e-12 of its
+  maximum, nrj itself (the log, which magnifies the relative error where
+  ||E|| passes its minimum, 0.5 % of its maximum) at 5e-11, and nrjy -- Ey is
+  round-off on both sides, the initial condition having no y dependence --
+  absolutely at 1e-12 of max nrjx.
 * 64^4 two-stream box (x, y in [0, 10 pi), v in [-8, 8], dt = 0.05): the
   reference's TSI20 initial condition (VMO/Init.cpp:247-277) on that grid,
   8 steps against the oracle at 1e-12, then 400 steps on the GPU alone,
@@ -32,7 +46,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_strong_landau_32_stated_dt_is_a_cfl_error(vlp):
-    args = sld_box(32, vmax=6.0, dt=0.1, nbiter=100, idcase=10, lx=4 * math.pi)
+    args = sld_box(32, vmax=6.0, dt=0.1, nbiter=100, idcase=vlp.CASE_STRONG_LANDAU, lx=4 * math.pi)
     with vlp.Vlp4d(vlp.VlpConfig.make(*args)) as sim:
         with pytest.raises(vlp.Vlp4dError) as ei:
             sim.step(1)
@@ -56,8 +70,43 @@ def _parity_run(vlp, orc, args, steps):
     assert np.max(np.abs(d["mass"])) < 16 * noise
 
 
-def test_strong_landau_32_twin_100_steps_matches_oracle(vlp, orc):
-    _parity_run(vlp, orc, sld_box(32, vmax=6.0, dt=0.05, nbiter=100, idcase=10, lx=4 * math.pi), 100)
+@pytest.mark.parametrize("idcase", [30, 10])
+def test_strong_landau_32_100_steps_matches_oracle(vlp, orc, idcase):
+    _parity_run(vlp, orc, sld_box(32, vmax=6.0, dt=0.05, nbiter=100, idcase=idcase, lx=4 * math.pi), 100)
+
+
+@pytest.mark.parametrize("n,idcase,lx,vmax", [((24, 20, 16, 18), 30, 4 * math.pi, 6.0),
+                                              ((20, 24, 18, 16), 31, 10 * math.pi, 8.0)])
+def test_extension_init_matches_oracle(vlp, orc, n, idcase, lx, vmax):
+    args = (list(n), [0.0, 0.0, -vmax, -vmax], [lx, lx, vmax, vmax], idcase, 0.05, 1)
+    with vlp.Vlp4d(vlp.VlpConfig.make(*args)) as sim:
+        f = sim.get_f()
+    assert relmax(f, orc.interior_of(orc.init_case(orc.make_config(*args)))) < 1e-14
+
+
+def test_two_stream_64_400_steps_match_oracle_run(vlp):
+    import os
+    from conftest import GOLDEN
+    ref = np.load(os.path.join(GOLDEN, "two_stream_64.npz"))
+    steps = 400
+    args = sld_box(64, vmax=8.0, dt=0.05, nbiter=steps, idcase=vlp.CASE_TWO_STREAM, lx=10 * math.pi)
+    errs = []
+    with vlp.Vlp4d(vlp.VlpConfig.make(*args)) as sim:
+        for k, s in enumerate(ref["steps"]):
+            sim.step(int(s) - sim.iter)
+            f = sim.get_f().reshape(64, 64, 64, 64)[::8, ::8, ::8, ::8]
+            errs.append(relmax(f, ref["f_sub"][k]))
+        d = sim.diags()
+    ser = {k: relmax(d[k], ref[k]) for k in ("nrj", "nrjx", "l2norm")}
+    ser["|E|"] = relmax(np.exp(d["nrj"]), np.exp(ref["nrj"]))
+    ser["nrjy abs / max nrjx"] = float(np.max(np.abs(d["nrjy"] - ref["nrjy"])) / np.max(np.abs(ref["nrjx"])))
+    print("two-stream 64^4 x400 vs oracle: f at steps", [int(s) for s in ref["steps"]], [f"{e:.1e}" for e in errs],
+          "series", {k: f"{v:.1e}" for k, v in ser.items()})
+    assert max(errs) < 1e-12
+    for k in ("nrjx", "l2norm", "|E|", "nrjy abs / max nrjx"):
+        assert ser[k] < 1e-12, k
+    assert ser["nrj"] < 5e-11
+    assert np.max(np.abs(d["mass"])) < 1e-10 and np.max(np.abs(ref["mass"])) < 1e-10
 
 
 def test_two_stream_64_matches_oracle(vlp, orc):
