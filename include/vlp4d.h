@@ -269,6 +269,20 @@ int vlp_adjoint_field_energy(vlp_ctx *ctx, int nsteps, int checkpoint_every, con
 int vlp_field_solve(const vlp_config *cfg, const void *f, int f_bits, double *phi, double *ex, double *ey,
                     double *diag, void *stream);
 
+/* vlp_field_solve without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight loops around a solver
+ * that owns its buffers).  No counterpart in the reference.  Same arguments
+ * and results; the scratch lives in a static device array of
+ * VLP_FIELD_SCRATCH_DOUBLES doubles (24 MiB) that each GPU holds until the
+ * process ends, and grids whose scratch exceeds it (about Nx*Ny*Nvy beyond
+ * 2.6e6) return VLP_E_SIZE.  The first call on a device looks up the array's
+ * address on that device; calls only enqueue work on `stream` (no
+ * synchronisation).  Calls on one GPU share the array: calls that overlap
+ * in time on the same GPU race. */
+#define VLP_FIELD_SCRATCH_DOUBLES (3u << 20)
+int vlp_field_solve_noalloc(const vlp_config *cfg, const void *f, int f_bits, double *phi, double *ex, double *ey,
+                            double *diag, void *stream);
+
 /* Ensembles: independent contexts (any grid sizes and precisions, one
  * device) advanced together, two steps of every member per launch of one
  * captured HIP graph (their kernels overlap on the GPU); an odd remainder

@@ -38,7 +38,7 @@ EXPORTS = [
     "vlp_spline_vxvy", "vlp_advect4d_poisson", "vlp_timer", "vlp_timer_label", "vlp_last_error",
     "vlp_kernel_variants", "vlp_autotune", "vlp_tuned_tile", "vlp_ensemble_create", "vlp_ensemble_step",
     "vlp_ensemble_size", "vlp_ensemble_destroy", "vlp_adjoint_step", "vlp_adjoint_steps", "vlp_create_lagrange", "vlp_write_fxvx",
-    "vlp_field_solve", "vlp_adjoint_field_energy",
+    "vlp_field_solve", "vlp_adjoint_field_energy", "vlp_field_solve_noalloc",
 ]
 
 
@@ -129,6 +129,7 @@ def load(path=LIB_PATH):
     lib.vlp_create_lagrange.argtypes = [P(VlpConfig), ctypes.c_int, P(vp)]
     lib.vlp_write_fxvx.argtypes = [vp, ctypes.c_char_p]
     lib.vlp_field_solve.argtypes = [P(VlpConfig), vp, ctypes.c_int, vp, vp, vp, vp, vp]
+    lib.vlp_field_solve_noalloc.argtypes = [P(VlpConfig), vp, ctypes.c_int, vp, vp, vp, vp, vp]
     _lib = lib
     return lib
 
@@ -155,6 +156,16 @@ def trig_interp_matrix(n, m):
     s = np.arange(m)[:, None] / m - np.arange(n)[None, :] / n
     k = np.arange(1, n // 2)
     return (1.0 + 2.0 * np.cos(2 * np.pi * s[..., None] * k).sum(axis=-1) + np.cos(np.pi * n * s)) / n
+
+
+def field_solve_noalloc(cfg, f_ptr, f_bits, phi_ptr, ex_ptr, ey_ptr, diag_ptr, stream=None):
+    """vlp_field_solve_noalloc: the same field solve with no device
+    allocation (scratch in the library's static device array); it only
+    enqueues the work on `stream`."""
+    lib = load()
+    rc = lib.vlp_field_solve_noalloc(ctypes.byref(cfg), f_ptr, f_bits, phi_ptr, ex_ptr, ey_ptr, diag_ptr, stream)
+    if rc:
+        raise Vlp4dError(rc, lib.vlp_last_error(None).decode())
 
 
 def _dptr(a):
