@@ -34,7 +34,7 @@ def _run(cmd):
 
 
 def build(verbose=True):
-    srcs = [os.path.join(CSRC, n) for n in ("spline.hip", "xy_fused.hip", "advect.hip", "advect_stream.hip", "adjoint.hip", "lagrange.hip", "field.hip", "field_solve.hip", "field_solve_noalloc.hip", "vlp4d_api.cpp")]
+    srcs = [os.path.join(CSRC, n) for n in ("spline.hip", "spline_mfma.hip", "xy_fused.hip", "advect.hip", "advect_stream.hip", "adjoint.hip", "lagrange.hip", "field.hip", "field_solve.hip", "field_solve_noalloc.hip", "vlp4d_api.cpp")]
     objs = []
     procs = []
     for src in srcs:

@@ -58,6 +58,9 @@ struct vlp_ctx {
   hipEvent_t ev[8] = {};
   int ys_tvx = 0, ys_nb = 0;  // 4-D advection tile chosen by vlp_autotune (0: preference order)
   vlp_ensemble *owner = nullptr;  // the ensemble whose captured graph holds this context's buffers
+  // MFMA form of the vx / vy spline passes (VLP4D_SPLINE_MFMA=1 at creation):
+  // the line operators of lengths N2 and N3
+  void *mf_m[2] = {};
   // adjoint of one step (allocated on first use)
   double *adj_c4s = nullptr, *adj_blk = nullptr, *adj_tmp = nullptr, *adj_cxy = nullptr;
   double *adj_E = nullptr, *adj_rho = nullptr, *adj_k = nullptr, *adj_g = nullptr, *adj_grad = nullptr;
@@ -140,7 +143,7 @@ void free_all(vlp_ctx *c) {
                   c->fb.rho,   c->fb.ex,    c->fb.ey,     c->fb.exy, c->fb.spec1, c->fb.spec2, c->fb.twx,  c->fb.twy,
                   c->fb.diag_part, c->fb.rho_tmp, c->fb.diag, c->d_err, c->adj_c4s, c->adj_blk, c->adj_tmp,
                   c->adj_cxy, c->adj_E, c->adj_rho, c->adj_k, c->adj_g, c->adj_grad, c->adj_Mt[0], c->adj_Mt[1],
-                  c->adj_Mt[2], c->adj_Mt[3], c->lag_tab};
+                  c->adj_Mt[2], c->adj_Mt[3], c->lag_tab, c->mf_m[0], c->mf_m[1]};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_err) (void)hipHostFree(c->h_err);
@@ -212,8 +215,15 @@ vlp::LineMap vy_map(const Geom &g) {
 template <typename T> int run_spline_vxvy(vlp_ctx *c) {
   const Geom &g = c->g;
   const long F0 = g.n[0] + 6, F1 = g.n[1] + 6, F2 = g.n[2] + 6;
-  vlp::launch_spline_line<T>((const T *)c->cxy, (T *)c->tv, vx_map(g), c->stream);
   T *c4 = (T *)c->c4 + 1 + F0 * (1 + F1 * (1 + F2 * 1));
+  if (c->mf_m[0]) {  // matrix products on the MFMA units
+    vlp::launch_spline_mfma<T>((const T *)c->cxy, (T *)c->tv, vx_map(g), (const T *)c->mf_m[0],
+                               vlp::spline_mfma_kp(g.n[2]), c->stream);
+    vlp::launch_spline_mfma<T>((const T *)c->tv, c4, vy_map(g), (const T *)c->mf_m[1], vlp::spline_mfma_kp(g.n[3]),
+                               c->stream);
+    return VLP_OK;
+  }
+  vlp::launch_spline_line<T>((const T *)c->cxy, (T *)c->tv, vx_map(g), c->stream);
   vlp::launch_spline_line<T>((const T *)c->tv, c4, vy_map(g), c->stream);
   return VLP_OK;
 }
@@ -600,6 +610,23 @@ static int create_ctx(const vlp_config *cfg_in, int precision, int device, int s
       hipMemset(c->fb.ey, 0, N0 * N1 * sizeof(double)) != hipSuccess ||
       hipMemset(c->fb.exy, 0, 2 * N0 * N1 * sizeof(double)) != hipSuccess)
     return fail(set_err(nullptr, VLP_E_HIP, "device initialisation failed"));
+  const char *mfe = getenv("VLP4D_SPLINE_MFMA");
+  if (!lag && mfe && atoi(mfe) == 1 && vlp::spline_mfma_eligible(vx_map(c->g), c->esz) &&
+      vlp::spline_mfma_eligible(vy_map(c->g), c->esz)) {
+    for (int k = 0; k < 2; k++) {
+      const int N = c->g.n[2 + k];
+      const size_t ne = vlp::spline_mfma_matrix_elems(N);
+      double *work = nullptr;
+      if (hipMalloc(&c->mf_m[k], ne * c->esz) != hipSuccess ||
+          hipMalloc((void **)&work, ((size_t)N * N + ne) * sizeof(double)) != hipSuccess)
+        return fail(set_err(nullptr, VLP_E_HIP, "hipMalloc failed (MFMA spline operator)"));
+      if (precision == VLP_FP64) vlp::build_spline_mfma_matrix<double>(N, (double *)c->mf_m[k], work, c->stream);
+      else vlp::build_spline_mfma_matrix<float>(N, (float *)c->mf_m[k], work, c->stream);
+      const bool ok = hipStreamSynchronize(c->stream) == hipSuccess && hipGetLastError() == hipSuccess;
+      (void)hipFree(work);
+      if (!ok) return fail(set_err(nullptr, VLP_E_HIP, "MFMA spline operator"));
+    }
+  }
   // init(): VMO/Init.cpp:370-372 then VMO/Vlp4d.cpp:30-34
   const auto t0 = std::chrono::steady_clock::now();
   if (lag) {
@@ -868,6 +895,7 @@ int vlp_kernel_variants(const vlp_ctx *c) {
   if ((bits & 128) && c->ys_tvx) bits = (bits & ~(15 << 8)) | (c->ys_tvx << 8);
   if (vlp::spline_tile_width(vx_map(c->g), c->esz) == 128) bits |= 1 << 13;
   if (vlp::spline_tile_width(vy_map(c->g), c->esz) == 128) bits |= 1 << 14;
+  if (c->mf_m[0]) bits = (bits & ~(3 << 13)) | (1 << 15);
   return bits;
 }
 

@@ -53,6 +53,16 @@ struct LineMap {
 template <typename T> void launch_init_case(const Geom &g, T *f, hipStream_t s);
 template <typename T> void launch_spline_x(const Geom &g, const T *f, T *tx, hipStream_t s);
 template <typename T> void launch_spline_line(const T *in, T *out, const LineMap &m, hipStream_t s);
+// The strided spline passes as MFMA matrix products (spline_mfma.hip):
+// OUT = M IN per outer index, M the exact (N+4) x N line operator, row-major
+// with Kp = spline_mfma_kp(N) columns, spline_mfma_matrix_elems(N) values
+// (rows padded to a multiple of 16), built by build_spline_mfma_matrix from
+// the double line kernel (work: N*N + spline_mfma_matrix_elems(N) doubles).
+bool spline_mfma_eligible(const LineMap &m, size_t tsz);
+template <typename T> void launch_spline_mfma(const T *in, T *out, const LineMap &m, const T *Mm, int Kp, hipStream_t s);
+int spline_mfma_kp(int N);
+size_t spline_mfma_matrix_elems(int N);
+template <typename T> void build_spline_mfma_matrix(int N, T *Mm, double *work, hipStream_t s);
 // Tile width (64 or 128 positions) of the LDS-tiled kernel launch_spline_line
 // picks for this line family and storage size, 0 when it takes a line kernel.
 int spline_tile_width(const LineMap &m, size_t tsz);
@@ -99,7 +109,8 @@ long l2_partials(const Geom &g);
 // advection arithmetic, bit 7: the y-streaming 4-D advection replaces the
 // vx-blocked one, bits 8-11: its vx tile, bit 12: the fused xy spline + 2-D
 // advection kernel replaces the x, y and advect2d kernels, bits 13 / 14: the
-// vx / vy spline pass runs 128-position tiles (four lanes per 128-point line).
+// vx / vy spline pass runs 128-position tiles (four lanes per 128-point line),
+// bit 15: the vx and vy passes run as MFMA matrix products (VLP4D_SPLINE_MFMA=1).
 int kernel_variants(const Geom &g, bool fp32_storage, bool arith32);
 
 // Adjoint of one step (adjoint.hip), FP64.
