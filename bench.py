@@ -173,30 +173,20 @@ def ensemble_bench(pkg, steps, seed=20261202):
     64^4, f stored in float with float interpolation arithmetic, perturbation
     amplitude alpha ~ U[0.01, 0.5] and wave number k in {0.3, 0.5} from a
     seeded generator, box [0, 2 pi / k)^2 x [-6, 6]^2, dt = dx / (2 V),
-    autotuned 4-D advection tiles.  Reports solves per second (a solve = the
-    given number of steps of one member) and the same members stepped one
-    after another for comparison."""
+    autotuned 4-D advection tiles and spline kernels.  Reports solves per
+    second (a solve = the given number of steps of one member) and the same
+    members stepped one after another for comparison."""
     import numpy as np
-    rng = np.random.default_rng(seed)
-    shapes = [(32, 32, 32, 32)] * 16 + [(48, 48, 64, 64)] * 16 + [(64, 64, 64, 64)] * 16
     members, cfgs = [], []
-    for n in shapes:
-        alpha = float(rng.uniform(0.01, 0.5))
-        k = float(rng.choice([0.3, 0.5]))
-        lx, vmax = 2 * math.pi / k, 6.0
-        dt = 0.5 * (lx / max(n[0], n[1])) / vmax
-        cfg = pkg.VlpConfig.make(list(n), [0.0, 0.0, -vmax, -vmax], [lx, lx, vmax, vmax], 10, dt, 2 * steps + 2)
+    for n, alpha, k in pkg.landau_ensemble_params(seed):
+        cargs, f0 = pkg.landau_member_setup(n, alpha, k)
+        cfg = pkg.VlpConfig.make(*cargs, 2 * steps + 2)
         sim = pkg.Vlp4d(cfg, precision=pkg.FP32_ACC)
-        x = np.arange(n[0]) * (lx / n[0])
-        y = np.arange(n[1]) * (lx / n[1])
-        vx = -vmax + np.arange(n[2]) * (2 * vmax / n[2])
-        vy = -vmax + np.arange(n[3]) * (2 * vmax / n[3])
-        sp = 1 + alpha * np.cos(k * x)[None, :] * np.cos(k * y)[:, None]           # [y][x]
-        mv = np.exp(-0.5 * (vx[None, :] ** 2 + vy[:, None] ** 2)) / (2 * math.pi)  # [vy][vx]
-        sim.set_f(mv[:, :, None, None] * sp[None, None, :, :])
+        sim.set_f(f0)
         sim.autotune()
         members.append(sim)
         cfgs.append((n, alpha, k))
+    shapes = [c[0] for c in cfgs]
     # the same members stepped one after another (eager), then the ensemble
     t0 = time.perf_counter()
     for m in members:

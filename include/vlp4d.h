@@ -201,15 +201,18 @@ int vlp_kernel_variants(const vlp_ctx *ctx);
  * ctx is NULL). */
 const char *vlp_last_error(const vlp_ctx *ctx);
 
-/* Autotuner of the 4-D advection's workgroup shape (vx tile, LDS plane
- * buffers): every shape the geometry admits is timed on the context's own
- * buffers and the fastest is used by its later steps; csv_path (optional)
- * receives the timings.  Follows the reference's tile tuner, which scans the
- * candidate tiles, keeps the minimum and logs a CSV
- * (miniapps/vlp4d_mpi/kokkos/Tile_size_tuning.hpp:51-96,127-148; per-kernel
- * registration miniapps/vlp4d_mpi/kokkos/Tuning.hpp:26-65).  The timed launches write F of the next
- * step, which that step overwrites.  The VLP4D_YS_* environment overrides
- * disable it. */
+/* Autotuner: the 4-D advection's workgroup shape (vx tile, LDS plane
+ * buffers) and the kernel of the vx and of the vy spline pass (128- or
+ * 64-position LDS tiles, or the line kernels): every choice the geometry
+ * admits is timed on the context's own buffers and the fastest is used by its
+ * later steps; csv_path (optional) receives the timings, one row per choice
+ * with the chosen ones marked.  Follows the reference's tile tuner, which
+ * scans the candidate tiles of every registered kernel, keeps the minimum and
+ * logs a CSV (miniapps/vlp4d_mpi/kokkos/Tile_size_tuning.hpp:51-96,127-148;
+ * per-kernel registration miniapps/vlp4d_mpi/kokkos/Tuning.hpp:26-65).  The
+ * timed launches write F of the next step and the coefficient buffers, which
+ * that step overwrites.  The VLP4D_YS_* (VLP4D_SPLINE_TILE/_WIDE) environment
+ * overrides pin the advection (spline) choice. */
 int vlp_autotune(vlp_ctx *ctx, const char *csv_path);
 /* The tuned (vx tile, plane buffers), 0 and 0 before vlp_autotune. */
 int vlp_tuned_tile(const vlp_ctx *ctx, int *vx_tile, int *plane_buffers);

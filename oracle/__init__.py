@@ -277,6 +277,13 @@ class State:
     def f_interior(self):
         return np.ascontiguousarray(interior_of(self.fn()))
 
+    def set_f(self, f):
+        """Replace the interior of f^n (x fastest, [vy][vx][y][x]); the next
+        step starts from it (its halo is rebuilt by the exchange)."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.shape == interior_shape(self.c)
+        self.lib.vo_set_interior(ctypes.byref(self.c), self.lib.vo_fn(self.h), _p(f))
+
     def field(self, which):
         n = self.c.nxmax[0] * self.c.nxmax[1]
         p = self.lib.vo_field(self.h, which)

@@ -14,6 +14,7 @@ The directory name is not a Python identifier; import it with
 ``importlib`` (see ``__graft_entry__.py``) or load ``libvlp4d.so`` directly.
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -141,6 +142,38 @@ def field_solve(cfg, f_ptr, f_bits, phi_ptr, ex_ptr, ey_ptr, diag_ptr, stream=No
     rc = lib.vlp_field_solve(ctypes.byref(cfg), f_ptr, f_bits, phi_ptr, ex_ptr, ey_ptr, diag_ptr, stream)
     if rc:
         raise Vlp4dError(rc, lib.vlp_last_error(None).decode())
+
+
+ENSEMBLE_SHAPES = [(32, 32, 32, 32)] * 16 + [(48, 48, 64, 64)] * 16 + [(64, 64, 64, 64)] * 16
+
+
+def landau_member_setup(n, alpha, k, vmax=6.0):
+    """(config arguments, initial f) of one ensemble member: Landau damping
+    f0 = (1 + alpha cos(k x) cos(k y)) exp(-|v|^2 / 2) / (2 pi) on the box
+    [0, 2 pi / k)^2 x [-vmax, vmax]^2, dt = dx / (2 vmax); returns
+    ((n, minphy, maxphy, idcase, dt), f0 as [vy][vx][y][x])."""
+    lx = 2 * math.pi / k
+    dt = 0.5 * (lx / max(n[0], n[1])) / vmax
+    x = np.arange(n[0]) * (lx / n[0])
+    y = np.arange(n[1]) * (lx / n[1])
+    vx = -vmax + np.arange(n[2]) * (2 * vmax / n[2])
+    vy = -vmax + np.arange(n[3]) * (2 * vmax / n[3])
+    sp = 1 + alpha * np.cos(k * x)[None, :] * np.cos(k * y)[:, None]           # [y][x]
+    mv = np.exp(-0.5 * (vx[None, :] ** 2 + vy[:, None] ** 2)) / (2 * math.pi)  # [vy][vx]
+    return (list(n), [0.0, 0.0, -vmax, -vmax], [lx, lx, vmax, vmax], 10, dt), mv[:, :, None, None] * sp[None, None]
+
+
+def landau_ensemble_params(seed, shapes=None):
+    """The batched Landau workload: for each grid (16 x 32^4, 16 x 48^2 x 64^2,
+    16 x 64^4 by default) a perturbation amplitude alpha ~ U[0.01, 0.5] and a
+    wave number k in {0.3, 0.5}, drawn in order from a seeded generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for n in (shapes or ENSEMBLE_SHAPES):
+        alpha = float(rng.uniform(0.01, 0.5))
+        k = float(rng.choice([0.3, 0.5]))
+        out.append((tuple(n), alpha, k))
+    return out
 
 
 def trig_interp_matrix(n, m):

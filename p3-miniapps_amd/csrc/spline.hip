@@ -401,11 +401,12 @@ static void launch_tile(const T *in, T *out, const LineMap &m, int P, hipStream_
 // 512), the tile in LDS.  Wide tiles move twice the bytes per page a tile
 // touches: each tile row sits on its own memory page when the line stride is
 // large, and the address translations per byte halve.
-static int tile_chunk(const LineMap &m, size_t tsz, int &P, int &XT) {
+static int tile_chunk(const LineMap &m, size_t tsz, int &P, int &XT, int mode = SPLINE_AUTO) {
+  if (mode == SPLINE_LINES) return 0;
   if (const char *e = getenv("VLP4D_SPLINE_TILE"))
     if (atoi(e) == 0) return 0;
   if (m.is0 != 1 || m.os0 != 1) return 0;
-  int wide = 1;  // VLP4D_SPLINE_WIDE=0: 64-position tiles only
+  int wide = mode == SPLINE_TILE64 ? 0 : 1;  // VLP4D_SPLINE_WIDE=0: 64-position tiles only
   if (const char *e = getenv("VLP4D_SPLINE_WIDE")) wide = atoi(e);
   if (wide && m.N % 32 == 0 && (m.N / 32) * 128 <= 512 && ((m.N / 32) & (m.N / 32 - 1)) == 0 &&
       (size_t)(m.N + 4) * 128 * tsz <= 160 * 1024 && (m.ils * tsz) % 16 == 0 && (m.is1 * tsz) % 16 == 0 &&
@@ -427,11 +428,11 @@ static int tile_chunk(const LineMap &m, size_t tsz, int &P, int &XT) {
   return C;
 }
 
-template <typename T> void launch_spline_line(const T *in, T *out, const LineMap &m, hipStream_t s) {
+template <typename T> void launch_spline_line(const T *in, T *out, const LineMap &m, hipStream_t s, int mode) {
   int P, logP, C;
   {
     int Pt, XT;
-    switch (tile_chunk(m, sizeof(T), Pt, XT)) {
+    switch (tile_chunk(m, sizeof(T), Pt, XT, mode)) {
       case 10: return launch_tile<T, 10, 64>(in, out, m, Pt, s);
       case 11: return launch_tile<T, 11, 64>(in, out, m, Pt, s);
       case 12: return launch_tile<T, 12, 64>(in, out, m, Pt, s);
@@ -463,9 +464,9 @@ template <typename T> void launch_spline_line(const T *in, T *out, const LineMap
 }
 #undef VLP_LINE_CASE
 
-int spline_tile_width(const LineMap &m, size_t tsz) {
+int spline_tile_width(const LineMap &m, size_t tsz, int mode) {
   int P, XT;
-  return tile_chunk(m, tsz, P, XT) ? XT : 0;
+  return tile_chunk(m, tsz, P, XT, mode) ? XT : 0;
 }
 
 int spline_variant_bits(const Geom &g) {
@@ -482,7 +483,7 @@ int spline_variant_bits(const Geom &g) {
 
 template void launch_spline_x<double>(const Geom &, const double *, double *, hipStream_t);
 template void launch_spline_x<float>(const Geom &, const float *, float *, hipStream_t);
-template void launch_spline_line<double>(const double *, double *, const LineMap &, hipStream_t);
-template void launch_spline_line<float>(const float *, float *, const LineMap &, hipStream_t);
+template void launch_spline_line<double>(const double *, double *, const LineMap &, hipStream_t, int);
+template void launch_spline_line<float>(const float *, float *, const LineMap &, hipStream_t, int);
 
 }  // namespace vlp

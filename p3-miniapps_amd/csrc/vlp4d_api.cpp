@@ -57,6 +57,7 @@ struct vlp_ctx {
   int tcalls[VLP_T_COUNT] = {};
   hipEvent_t ev[8] = {};
   int ys_tvx = 0, ys_nb = 0;  // 4-D advection tile chosen by vlp_autotune (0: preference order)
+  int sp_mode[2] = {vlp::SPLINE_AUTO, vlp::SPLINE_AUTO};  // vx / vy spline kernels chosen by vlp_autotune
   vlp_ensemble *owner = nullptr;  // the ensemble whose captured graph holds this context's buffers
   // MFMA form of the vx / vy spline passes (VLP4D_SPLINE_MFMA=1 at creation):
   // the line operators of lengths N2 and N3
@@ -223,8 +224,8 @@ template <typename T> int run_spline_vxvy(vlp_ctx *c) {
                                c->stream);
     return VLP_OK;
   }
-  vlp::launch_spline_line<T>((const T *)c->cxy, (T *)c->tv, vx_map(g), c->stream);
-  vlp::launch_spline_line<T>((const T *)c->tv, c4, vy_map(g), c->stream);
+  vlp::launch_spline_line<T>((const T *)c->cxy, (T *)c->tv, vx_map(g), c->stream, c->sp_mode[0]);
+  vlp::launch_spline_line<T>((const T *)c->tv, c4, vy_map(g), c->stream, c->sp_mode[1]);
   return VLP_OK;
 }
 
@@ -893,14 +894,15 @@ int vlp_kernel_variants(const vlp_ctx *c) {
   if (c->scheme == VLP_SCHEME_LAGRANGE) return vlp::lag_xy_chunks(c->g) > 0 ? 1 : 0;  // bit 0: fused x/y pass
   int bits = vlp::kernel_variants(c->g, c->precision != VLP_FP64, c->precision == VLP_FP32_ACC);
   if ((bits & 128) && c->ys_tvx) bits = (bits & ~(15 << 8)) | (c->ys_tvx << 8);
-  if (vlp::spline_tile_width(vx_map(c->g), c->esz) == 128) bits |= 1 << 13;
-  if (vlp::spline_tile_width(vy_map(c->g), c->esz) == 128) bits |= 1 << 14;
+  if (vlp::spline_tile_width(vx_map(c->g), c->esz, c->sp_mode[0]) == 128) bits |= 1 << 13;
+  if (vlp::spline_tile_width(vy_map(c->g), c->esz, c->sp_mode[1]) == 128) bits |= 1 << 14;
   if (c->mf_m[0]) bits = (bits & ~(3 << 13)) | (1 << 15);
   return bits;
 }
 
-// Autotuner of the 4-D advection's workgroup shape: every (vx tile, plane
-// buffers) this geometry admits is launched on the context's own buffers
+// Autotuner of the 4-D advection's workgroup shape and of the vx / vy spline
+// kernels: every (vx tile, plane buffers) this geometry admits, and every
+// distinct spline kernel choice, is launched on the context's own buffers
 // (one warm-up and three timed launches, HIP events on its stream) and the
 // fastest is kept for the context's later steps.  Same scan-and-keep-the-
 // minimum scheme as the reference's tile tuner
@@ -911,47 +913,99 @@ int vlp_autotune(vlp_ctx *c, const char *csv_path) {
   int rc = spline_only(c);
   if (rc) return rc;
   HIP_TRY(c, hipSetDevice(c->device));
-  if (getenv("VLP4D_YS_TVX") || getenv("VLP4D_YS_NBUF") || getenv("VLP4D_YSTREAM")) return VLP_OK;
+  // the VLP4D_YS_* / VLP4D_SPLINE_* environment overrides pin those choices
+  const bool ys_env = getenv("VLP4D_YS_TVX") || getenv("VLP4D_YS_NBUF") || getenv("VLP4D_YSTREAM");
+  const bool sp_env = getenv("VLP4D_SPLINE_TILE") || getenv("VLP4D_SPLINE_WIDE");
   const size_t tsz = c->precision == VLP_FP64 ? sizeof(double) : sizeof(float);
   struct Cand { int tvx, nb; float ms; };
   std::vector<Cand> cands;
   for (int tvx : {2, 4, 8})
     for (int nb : {2, 3})
-      if (vlp::ystream_eligible(c->g, tsz, tvx, nb)) cands.push_back({tvx, nb, 0.f});
-  if (cands.empty()) return VLP_OK;
-  int *d_scratch = nullptr;
-  HIP_TRY(c, hipMalloc((void **)&d_scratch, sizeof(int)));
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
-  for (auto &k : cands) {
-    if (!ok) break;
-    for (int it = 0; it < 4 && ok; it++) {
-      if (it == 1) ok = hipEventRecord(e0, c->stream) == hipSuccess;
-      if (c->precision == VLP_FP64)
-        vlp::launch_advect4d_ystream<double, double>(c->g, (const double *)c->c4, c->fb.exy, c->cfg.dt,
-                                                     (double *)c->fnext, d_scratch, c->stream, k.tvx, k.nb);
-      else if (c->precision == VLP_FP32)
-        vlp::launch_advect4d_ystream<float, double>(c->g, (const float *)c->c4, c->fb.exy, c->cfg.dt,
-                                                    (float *)c->fnext, d_scratch, c->stream, k.tvx, k.nb);
-      else
-        vlp::launch_advect4d_ystream<float, float>(c->g, (const float *)c->c4, c->fb.exy, c->cfg.dt,
-                                                   (float *)c->fnext, d_scratch, c->stream, k.tvx, k.nb);
-      ok = ok && hipGetLastError() == hipSuccess;
+      if (!ys_env && vlp::ystream_eligible(c->g, tsz, tvx, nb)) cands.push_back({tvx, nb, 0.f});
+  // spline passes: each distinct kernel choice (128- or 64-position tiles,
+  // line kernels) of the vx and the vy pass, on cxy -> tv and tv -> c4
+  // (both overwritten by the next step)
+  struct SCand { int pass, mode, width; float ms; };
+  std::vector<SCand> spl;
+  for (int pass = 0; pass < 2 && !c->mf_m[0] && !sp_env; pass++) {
+    const vlp::LineMap m = pass ? vy_map(c->g) : vx_map(c->g);
+    std::vector<int> seen;
+    for (int mode : {vlp::SPLINE_AUTO, vlp::SPLINE_TILE64, vlp::SPLINE_LINES}) {
+      const int w = vlp::spline_tile_width(m, c->esz, mode);
+      if (std::find(seen.begin(), seen.end(), w) != seen.end()) continue;
+      seen.push_back(w);
+      spl.push_back({pass, mode, w, 0.f});
     }
-    ok = ok && hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
-    float ms = 0.f;
-    ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
-    k.ms = ms / 3.f;
   }
-  if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
-  (void)hipFree(d_scratch);
-  if (!ok) return set_err(c, VLP_E_HIP, "autotune launch failed");
-  const Cand *best = &cands[0];
-  for (const auto &k : cands)
-    if (k.ms < best->ms) best = &k;
-  c->ys_tvx = best->tvx;
-  c->ys_nb = best->nb;
+  {
+    hipEvent_t a = nullptr, b = nullptr;
+    bool sok = hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess;
+    const long F0 = c->g.n[0] + 6, F1 = c->g.n[1] + 6, F2 = c->g.n[2] + 6;
+    const size_t c4off = 1 + F0 * (1 + F1 * (1 + F2 * 1));
+    for (auto &k : spl) {
+      if (!sok) break;
+      const vlp::LineMap m = k.pass ? vy_map(c->g) : vx_map(c->g);
+      for (int it = 0; it < 4 && sok; it++) {
+        if (it == 1) sok = hipEventRecord(a, c->stream) == hipSuccess;
+        if (c->precision == VLP_FP64)
+          vlp::launch_spline_line<double>(k.pass ? (const double *)c->tv : (const double *)c->cxy,
+                                          k.pass ? (double *)c->c4 + c4off : (double *)c->tv, m, c->stream, k.mode);
+        else
+          vlp::launch_spline_line<float>(k.pass ? (const float *)c->tv : (const float *)c->cxy,
+                                         k.pass ? (float *)c->c4 + c4off : (float *)c->tv, m, c->stream, k.mode);
+        sok = sok && hipGetLastError() == hipSuccess;
+      }
+      sok = sok && hipEventRecord(b, c->stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess;
+      float ms = 0.f;
+      sok = sok && hipEventElapsedTime(&ms, a, b) == hipSuccess;
+      k.ms = ms / 3.f;
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (!sok) return set_err(c, VLP_E_HIP, "autotune launch failed (spline)");
+    for (int pass = 0; pass < 2; pass++) {
+      const SCand *bs = nullptr;
+      for (const auto &k : spl)
+        if (k.pass == pass && (!bs || k.ms < bs->ms)) bs = &k;
+      if (bs) c->sp_mode[pass] = bs->mode;
+    }
+  }
+  const Cand *best = nullptr;
+  if (!cands.empty()) {
+    int *d_scratch = nullptr;
+    HIP_TRY(c, hipMalloc((void **)&d_scratch, sizeof(int)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    for (auto &k : cands) {
+      if (!ok) break;
+      for (int it = 0; it < 4 && ok; it++) {
+        if (it == 1) ok = hipEventRecord(e0, c->stream) == hipSuccess;
+        if (c->precision == VLP_FP64)
+          vlp::launch_advect4d_ystream<double, double>(c->g, (const double *)c->c4, c->fb.exy, c->cfg.dt,
+                                                       (double *)c->fnext, d_scratch, c->stream, k.tvx, k.nb);
+        else if (c->precision == VLP_FP32)
+          vlp::launch_advect4d_ystream<float, double>(c->g, (const float *)c->c4, c->fb.exy, c->cfg.dt,
+                                                      (float *)c->fnext, d_scratch, c->stream, k.tvx, k.nb);
+        else
+          vlp::launch_advect4d_ystream<float, float>(c->g, (const float *)c->c4, c->fb.exy, c->cfg.dt,
+                                                     (float *)c->fnext, d_scratch, c->stream, k.tvx, k.nb);
+        ok = ok && hipGetLastError() == hipSuccess;
+      }
+      ok = ok && hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+      float ms = 0.f;
+      ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+      k.ms = ms / 3.f;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d_scratch);
+    if (!ok) return set_err(c, VLP_E_HIP, "autotune launch failed");
+    best = &cands[0];
+    for (const auto &k : cands)
+      if (k.ms < best->ms) best = &k;
+    c->ys_tvx = best->tvx;
+    c->ys_nb = best->nb;
+  }
   if (csv_path && *csv_path) {
     FILE *f = fopen(csv_path, "w");
     if (!f) return set_err(c, VLP_E_FILE, std::string("cannot write ") + csv_path);
@@ -959,6 +1013,9 @@ int vlp_autotune(vlp_ctx *c, const char *csv_path) {
     for (const auto &k : cands)
       fprintf(f, "advect4d,%d,%d,%d,%d,%d,%d,%d,%.6f,%d\n", c->g.n[0], c->g.n[1], c->g.n[2], c->g.n[3], c->precision,
               k.tvx, k.nb, k.ms, &k == best ? 1 : 0);
+    for (const auto &k : spl)
+      fprintf(f, "spline_%s,%d,%d,%d,%d,%d,%d,%d,%.6f,%d\n", k.pass ? "vy" : "vx", c->g.n[0], c->g.n[1], c->g.n[2],
+              c->g.n[3], c->precision, k.width, 0, k.ms, k.mode == c->sp_mode[k.pass] ? 1 : 0);
     fclose(f);
   }
   return VLP_OK;
@@ -1281,12 +1338,15 @@ struct vlp_ensemble {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<void *> cap_f;  // each member's f when the graph was captured
-  std::vector<int> cap_tile;  // and its 4-D advection tile (ys_tvx * 16 + ys_nb)
+  std::vector<int> cap_tile;  // and its tuned kernel choices (tile_key)
   double seconds = 0.;
   long steps = 0;
 };
 
 namespace {
+
+// The tuned kernel choices of a context, as one number.
+int tile_key(const vlp_ctx *c) { return ((c->ys_tvx * 16 + c->ys_nb) * 4 + c->sp_mode[0]) * 4 + c->sp_mode[1]; }
 
 void ensemble_free(vlp_ensemble *e) {
   if (e->exec) (void)hipGraphExecDestroy(e->exec);
@@ -1333,7 +1393,7 @@ int ensemble_capture(vlp_ensemble *e) {
   e->cap_tile.resize(e->m.size());
   for (size_t i = 0; i < e->m.size(); i++) {
     e->cap_f[i] = e->m[i]->f;
-    e->cap_tile[i] = e->m[i]->ys_tvx * 16 + e->m[i]->ys_nb;
+    e->cap_tile[i] = tile_key(e->m[i]);
   }
   return VLP_OK;
 }
@@ -1391,7 +1451,7 @@ int vlp_ensemble_step(vlp_ensemble *e, int nsteps) {
     // the graph holds each member's buffers and tile as captured
     bool fresh = e->exec != nullptr;
     for (size_t i = 0; i < e->m.size() && fresh; i++)
-      fresh = e->cap_f[i] == e->m[i]->f && e->cap_tile[i] == e->m[i]->ys_tvx * 16 + e->m[i]->ys_nb;
+      fresh = e->cap_f[i] == e->m[i]->f && e->cap_tile[i] == tile_key(e->m[i]);
     if (!fresh) {
       const int rc = ensemble_capture(e);
       if (rc) return rc;

@@ -52,7 +52,14 @@ struct LineMap {
 // interpolation weights and accumulation for float storage (else double).
 template <typename T> void launch_init_case(const Geom &g, T *f, hipStream_t s);
 template <typename T> void launch_spline_x(const Geom &g, const T *f, T *tx, hipStream_t s);
-template <typename T> void launch_spline_line(const T *in, T *out, const LineMap &m, hipStream_t s);
+// Kernel choice of a strided spline pass: SPLINE_AUTO the preference order
+// (128-position LDS tiles where N is a multiple of 32, else 64-position
+// tiles, else the line kernels), SPLINE_TILE64 64-position tiles (or the line
+// kernels), SPLINE_LINES the line kernels.  The VLP4D_SPLINE_* environment
+// variables override it.
+enum { SPLINE_AUTO = 0, SPLINE_TILE64 = 1, SPLINE_LINES = 2 };
+template <typename T>
+void launch_spline_line(const T *in, T *out, const LineMap &m, hipStream_t s, int mode = SPLINE_AUTO);
 // The strided spline passes as MFMA matrix products (spline_mfma.hip):
 // OUT = M IN per outer index, M the exact (N+4) x N line operator, row-major
 // with Kp = spline_mfma_kp(N) columns, spline_mfma_matrix_elems(N) values
@@ -65,7 +72,7 @@ size_t spline_mfma_matrix_elems(int N);
 template <typename T> void build_spline_mfma_matrix(int N, T *Mm, double *work, hipStream_t s);
 // Tile width (64 or 128 positions) of the LDS-tiled kernel launch_spline_line
 // picks for this line family and storage size, 0 when it takes a line kernel.
-int spline_tile_width(const LineMap &m, size_t tsz);
+int spline_tile_width(const LineMap &m, size_t tsz, int mode = SPLINE_AUTO);
 template <typename T>
 void launch_advect2d_rho(const Geom &g, const T *cxy, double dt_half, bool arith32, double *rho_part, int *err,
                         hipStream_t s);

@@ -6,8 +6,11 @@ replayed graph; every replay is checked against a twin set of contexts
 stepped eagerly with vlp_step, with eager work on the members between the
 replays (an odd step count, which runs its last step eagerly and leaves the
 members' buffers swapped, so the next replay needs a fresh capture).  The
-autotuner changes only the 4-D advection's workgroup shape, which does not
-change any cell's arithmetic: a tuned context gives the untuned one's result.
+autotuner changes the 4-D advection's workgroup shape, which does not
+change any cell's arithmetic, and the spline passes' kernels (128- or
+64-position LDS tiles or line kernels), which split the line recursions
+differently: a tuned context gives the untuned one's result to round-off
+(FP64) or float rounding (float storage).
 """
 import math
 import os
@@ -99,9 +102,55 @@ def test_autotune_keeps_results(vlp, tmp_path, prec):
         assert (a.kernel_variants() >> 8) & 15 == tvx
         rows = csv.read_text().strip().splitlines()
         assert rows[0].startswith("kernel,") and len(rows) >= 2
-        assert sum(int(r.split(",")[-1]) for r in rows[1:]) == 1
+        for fam in ("advect4d", "spline_vx", "spline_vy"):   # one choice per tuned kernel
+            assert sum(int(r.split(",")[-1]) for r in rows[1:] if r.split(",")[0] == fam) == 1, fam
         a.step(5)
         b.step(5)
-        assert relmax(a.get_f(), b.get_f()) < 1e-14
+        # the advection tile leaves every cell's arithmetic unchanged; the
+        # spline kernels split the line recursion differently (round-off,
+        # then float rounding of the stored coefficients)
+        bar = 1e-13 if prec == "FP64" else 1e-5
+        assert relmax(a.get_f(), b.get_f()) < bar
         da, db = a.diags(), b.diags()
-        assert relmax(da["nrj"], db["nrj"]) < 1e-14
+        assert relmax(da["nrj"], db["nrj"]) < bar
+
+
+def test_ensemble_workload_48_members_match_oracle(vlp, orc):
+    """The batched Landau workload itself: 48 members (16 x 32^4, 16 x 48^2 x
+    64^2, 16 x 64^4), alpha ~ U[0.01, 0.5], k in {0.3, 0.5} (seeded), f stored
+    in float with float interpolation arithmetic, autotuned 4-D advection
+    tiles and spline kernels, 4 steps as two graph replays; every member
+    against the FP64 oracle started from the member's own (float-rounded) f:
+    f within 5e-5 (relative max-norm) and the field energy W = exp(2 nrj)
+    within 1e-5 over the four steps -- the float-storage budget of
+    test_gpu_fp32.py for a few steps."""
+    steps = 4
+    params = vlp.landau_ensemble_params(20261202)
+    members, starts = [], []
+    for n, alpha, k in params:
+        cargs, f0 = vlp.landau_member_setup(n, alpha, k)
+        sim = vlp.Vlp4d(vlp.VlpConfig.make(*cargs, steps), precision=vlp.FP32_ACC)
+        sim.set_f(f0)
+        sim.autotune()
+        members.append(sim)
+        starts.append(sim.get_f())
+    with vlp.Ensemble(members) as ens:
+        ens.step(steps)
+    worst = {}
+    for sim, f0, (n, alpha, k) in zip(members, starts, params):
+        cargs, _ = vlp.landau_member_setup(n, alpha, k)
+        st = orc.State(orc.make_config(*cargs, steps))
+        st.set_f(f0)
+        st.step(steps)
+        ef = relmax(sim.get_f(), st.f_interior())
+        w, wr = np.exp(2 * sim.diags()["nrj"][1:]), np.exp(2 * st.diags()["nrj"][1:])
+        ew = relmax(w, wr)
+        st.close()
+        key = tuple(n)
+        worst[key] = max(worst.get(key, (0, 0))[0], ef), max(worst.get(key, (0, 0))[1], ew)
+        assert sim.iter == steps
+        assert ef < 5e-5 and ew < 1e-5, (n, alpha, k, ef, ew)
+    for key, (ef, ew) in worst.items():
+        print(f"ensemble members {key}: worst f {ef:.2e}, worst W {ew:.2e}", flush=True)
+    for sim in members:
+        sim.close()
