@@ -350,33 +350,83 @@ bool launch_advect4d<float>(const Geom &g, const float *c4, const double *ex, co
   return false;
 }
 
-// moments of a given f: rho partials (sum over vx) and L2 partials
-template <typename T>
+// moments of a given f: rho partials (sum over vx) and L2 partials.  A
+// thread takes VX consecutive x of one (y, vy) and walks vx; for VX > 1 with
+// one 16-byte load per vx and four independent sums.  A block covers 256
+// points (x, y, vy), so there are l2_partials(g) blocks as for the other
+// producers.
+template <typename T, int VX>
 __global__ __launch_bounds__(256) void k_moments(const T *__restrict__ f, Geom g, double *__restrict__ rho_part,
                                                  double *__restrict__ l2_part) {
   __shared__ double sh[8];
   const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long p0 = (long)blockIdx.x * 256 + (long)threadIdx.x * VX;  // first point (x, y, vy)
+  const long npt = (long)N0 * N1 * N3;
   double l2 = 0.;
-  if (t < (long)N0 * N1 * N3) {
-    const int ix = (int)(t % N0);
-    const long r = t / N0;
+  if (p0 < npt) {
+    const int ix = (int)(p0 % N0);
+    const long r = p0 / N0;
     const int iy = (int)(r % N1);
     const int ivy = (int)(r / N1);
-    double rho = 0.;
-    for (int ivx = 0; ivx < N2; ivx++) {
-      const double v = (double)f[ix + (long)N0 * (iy + (long)N1 * (ivx + (long)N2 * ivy))];
-      rho += v;
-      l2 += v * v;
+    const long plane = (long)N0 * N1;
+    const T *src = f + ix + plane * ((long)N2 * ivy) + (long)N0 * iy;
+    if constexpr (VX == 1) {  // one running sum per thread (measured fastest for double)
+      double rho = 0.;
+      for (int ivx = 0; ivx < N2; ivx++) {
+        const double v = (double)src[plane * ivx];
+        rho += v;
+        l2 += v * v;
+      }
+      rho_part[p0] = rho;
+    } else {
+      double rho[4][VX], sq[4][VX];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int j = 0; j < VX; j++) rho[u][j] = sq[u][j] = 0.;
+      int ivx = 0;
+      for (; ivx + 4 <= N2; ivx += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          typedef T vec_t __attribute__((ext_vector_type(VX)));
+          const vec_t w = *(const vec_t *)(src + plane * (ivx + u));
+#pragma unroll
+          for (int j = 0; j < VX; j++) {
+            const double d = (double)w[j];
+            rho[u][j] += d;
+            sq[u][j] += d * d;
+          }
+        }
+      }
+      for (; ivx < N2; ivx++)
+#pragma unroll
+        for (int j = 0; j < VX; j++) {
+          const double d = (double)src[plane * ivx + j];
+          rho[0][j] += d;
+          sq[0][j] += d * d;
+        }
+#pragma unroll
+      for (int j = 0; j < VX; j++) {
+        rho_part[p0 + j] = (rho[0][j] + rho[1][j]) + (rho[2][j] + rho[3][j]);
+        l2 += (sq[0][j] + sq[1][j]) + (sq[2][j] + sq[3][j]);
+      }
     }
-    rho_part[ix + (long)N0 * (iy + (long)N1 * ivy)] = rho;
   }
   const double tot = block_sum(l2, sh);
   if (threadIdx.x == 0) l2_part[blockIdx.x] = tot;
 }
 
 template <typename T> void launch_moments(const Geom &g, const T *f, double *rho_part, double *l2_part, hipStream_t s) {
-  hipLaunchKernelGGL(k_moments<T>, dim3((unsigned)l2_partials(g)), dim3(256), 0, s, f, g, rho_part, l2_part);
+  // float: four consecutive x per thread (one 16-byte load per vx) where the
+  // rows hold whole 16-byte chunks (measured at 128^4: 0.22 ms against 0.32
+  // for one x per thread); double: one x and one running sum per thread
+  // (0.39 ms against 0.42 for two x per thread and 0.41 for four sums)
+  constexpr int VX = sizeof(T) == sizeof(float) ? 4 : 1;
+  if (VX > 1 && g.n[0] % VX == 0)
+    hipLaunchKernelGGL((k_moments<T, VX>), dim3((unsigned)l2_partials(g)), dim3(256 / VX), 0, s, f, g, rho_part,
+                       l2_part);
+  else
+    hipLaunchKernelGGL((k_moments<T, 1>), dim3((unsigned)l2_partials(g)), dim3(256), 0, s, f, g, rho_part, l2_part);
 }
 
 // ---------------------------------------------------------------------------

@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void k_spline_line_scan(const T *__restrict__ 
 typedef __attribute__((address_space(3))) void lds_void_t;
 
 template <typename T, int C, int XT>
-__global__ __launch_bounds__(512) void k_spline_tile(const T *__restrict__ in, T *__restrict__ out, LineMap m, int P,
+__global__ __launch_bounds__(XT == 256 ? 1024 : 512) void k_spline_tile(const T *__restrict__ in, T *__restrict__ out, LineMap m, int P,
                                                      SplineConsts sc, double aC, double aN3) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *tile = reinterpret_cast<T *>(smem_raw);  // [N+4][XT]
@@ -406,8 +406,17 @@ static int tile_chunk(const LineMap &m, size_t tsz, int &P, int &XT, int mode = 
   if (const char *e = getenv("VLP4D_SPLINE_TILE"))
     if (atoi(e) == 0) return 0;
   if (m.is0 != 1 || m.os0 != 1) return 0;
-  int wide = mode == SPLINE_TILE64 ? 0 : 1;  // VLP4D_SPLINE_WIDE=0: 64-position tiles only
+  // VLP4D_SPLINE_WIDE=0: 64-position tiles only; =2: 256-position tiles
+  // where they fit (float storage at N = 128: 135 KB of LDS, 1024 lanes)
+  int wide = mode == SPLINE_TILE64 ? 0 : mode == SPLINE_TILE256 ? 2 : 1;
   if (const char *e = getenv("VLP4D_SPLINE_WIDE")) wide = atoi(e);
+  if (wide == 2 && m.N % 32 == 0 && (m.N / 32) * 256 <= 1024 && ((m.N / 32) & (m.N / 32 - 1)) == 0 &&
+      (size_t)(m.N + 4) * 256 * tsz <= 160 * 1024 && (m.ils * tsz) % 16 == 0 && (m.is1 * tsz) % 16 == 0 &&
+      (m.ios * tsz) % 16 == 0 && ((m.is1 == m.X0 ? m.X0 * m.X1 : m.X0) * (long)tsz) % 16 == 0) {
+    P = m.N / 32;
+    XT = 256;
+    return 32;
+  }
   if (wide && m.N % 32 == 0 && (m.N / 32) * 128 <= 512 && ((m.N / 32) & (m.N / 32 - 1)) == 0 &&
       (size_t)(m.N + 4) * 128 * tsz <= 160 * 1024 && (m.ils * tsz) % 16 == 0 && (m.is1 * tsz) % 16 == 0 &&
       (m.ios * tsz) % 16 == 0 && ((m.is1 == m.X0 ? m.X0 * m.X1 : m.X0) * (long)tsz) % 16 == 0) {
@@ -440,7 +449,9 @@ template <typename T> void launch_spline_line(const T *in, T *out, const LineMap
       case 14: return launch_tile<T, 14, 64>(in, out, m, Pt, s);
       case 15: return launch_tile<T, 15, 64>(in, out, m, Pt, s);
       case 16: return launch_tile<T, 16, 64>(in, out, m, Pt, s);
-      case 32: return launch_tile<T, 32, 128>(in, out, m, Pt, s);
+      case 32:
+        if (XT == 256) return launch_tile<T, 32, 256>(in, out, m, Pt, s);
+        return launch_tile<T, 32, 128>(in, out, m, Pt, s);
       default: break;
     }
   }

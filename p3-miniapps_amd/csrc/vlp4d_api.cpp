@@ -930,7 +930,7 @@ int vlp_autotune(vlp_ctx *c, const char *csv_path) {
   for (int pass = 0; pass < 2 && !c->mf_m[0] && !sp_env; pass++) {
     const vlp::LineMap m = pass ? vy_map(c->g) : vx_map(c->g);
     std::vector<int> seen;
-    for (int mode : {vlp::SPLINE_AUTO, vlp::SPLINE_TILE64, vlp::SPLINE_LINES}) {
+    for (int mode : {vlp::SPLINE_AUTO, vlp::SPLINE_TILE256, vlp::SPLINE_TILE64, vlp::SPLINE_LINES}) {
       const int w = vlp::spline_tile_width(m, c->esz, mode);
       if (std::find(seen.begin(), seen.end(), w) != seen.end()) continue;
       seen.push_back(w);

@@ -55,9 +55,10 @@ template <typename T> void launch_spline_x(const Geom &g, const T *f, T *tx, hip
 // Kernel choice of a strided spline pass: SPLINE_AUTO the preference order
 // (128-position LDS tiles where N is a multiple of 32, else 64-position
 // tiles, else the line kernels), SPLINE_TILE64 64-position tiles (or the line
-// kernels), SPLINE_LINES the line kernels.  The VLP4D_SPLINE_* environment
+// kernels), SPLINE_LINES the line kernels, SPLINE_TILE256 256-position tiles
+// where they fit in LDS (else the AUTO choice).  The VLP4D_SPLINE_* environment
 // variables override it.
-enum { SPLINE_AUTO = 0, SPLINE_TILE64 = 1, SPLINE_LINES = 2 };
+enum { SPLINE_AUTO = 0, SPLINE_TILE64 = 1, SPLINE_LINES = 2, SPLINE_TILE256 = 3 };
 template <typename T>
 void launch_spline_line(const T *in, T *out, const LineMap &m, hipStream_t s, int mode = SPLINE_AUTO);
 // The strided spline passes as MFMA matrix products (spline_mfma.hip):

@@ -86,3 +86,19 @@ def test_mfma_spline_float_storage_against_scan(vlp, monkeypatch, n):
     print(f"grid {n} float storage: C4 {e_c4:.2e}, f after 20 steps {e_f:.2e}, W {e_w:.2e}", flush=True)
     assert e_c4 < 1e-6
     assert e_f < 1e-5 and e_w < 1e-5
+
+
+@pytest.mark.parametrize("n", [(16, 12, 128, 10), (16, 10, 12, 128), (32, 24, 128, 128)])
+def test_256_position_tiles_equal_128_position_tiles(vlp, monkeypatch, n):
+    """Float storage: 256-position LDS tiles (VLP4D_SPLINE_WIDE=2, 1024 lanes)
+    run the same line recursions as the 128-position tiles (four lanes of 32
+    points per 128-point line), only grouped differently: bit-identical."""
+    args = _args(n, steps=3)
+    out = {}
+    for wide in ("1", "2"):
+        monkeypatch.setenv("VLP4D_SPLINE_WIDE", wide)
+        with vlp.Vlp4d(vlp.VlpConfig.make(*args), precision=vlp.FP32_ACC) as sim:
+            sim.step(3)
+            out[wide] = (sim.get_f(), sim.diags()["nrj"])
+    np.testing.assert_array_equal(out["1"][0], out["2"][0])
+    np.testing.assert_array_equal(out["1"][1], out["2"][1])
