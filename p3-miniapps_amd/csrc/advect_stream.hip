@@ -32,7 +32,8 @@ namespace {
 
 // One open cell: B-spline weights along x, y, vx, vy ([dim][k]), running
 // sum, and the packed stencil offsets (ox+1) | (ovx+1) << 1 | (ovy+1) << 2,
-// or -1 when the cell takes no streamed contribution.  (Keeping the four y
+// or -1 when the cell takes no streamed contribution (with the same key as
+// a bit, so that the four cells' keys compare by two bit operations).  (Keeping the four y
 // weights rather than re-evaluating the one a plane needs from the
 // fractional y position measured 1 % faster with double arithmetic, at 246
 // registers.)
@@ -40,6 +41,7 @@ template <typename A> struct OpenCell {
   A w[4][4];
   A acc;
   int key;
+  int kbit;  // 1 << key, or 0 without a streamed contribution
 };
 
 typedef __attribute__((address_space(3))) void lds_void;
@@ -72,6 +74,11 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   T *slab;         // NB buffers of slab_bytes
   double *eb;      // [NB][3][EPR][2]
   double *kconst;  // LDS: domain constants of the foot computation (KC_* layout)
+  // LDS: per y-step values uniform in the workgroup, [q][QT]: the y of cell
+  // q, computeFeet's E row ip1 (as a double) and its three y Lagrange weights
+  double *qtab;
+  static constexpr int QT = 5;
+  T *fbase;  // f^{n+1} at (x, y = 0, vx, vy) of this lane
   static constexpr int KC_MIN = 0, KC_INV = 4, KC_LMIN = 8, KC_LMAX = 12, KC_VHI = 16, KC_N = 18;
   int slab_bytes;  // whole 1 KB blocks
   int EPR, cpr;    // E points per LDS row (whole 64-point chunks), chunks per row
@@ -100,8 +107,7 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   // E row ip1 of the cell y (computeFeet: round of the half-step y) - 1;
   // uniform in the workgroup.
   __device__ __forceinline__ int erow0(int y) const {
-    const double yy = g.minPhy[1] + y * g.dx[1];
-    return __builtin_amdgcn_readfirstlane((int)(round(half_y(g, yy, vy, dt)) - 1));
+    return __builtin_amdgcn_readfirstlane((int)qtab[QT * y + 1]);
   }
 
   // Start moving step q's slab and E rows into LDS buffer b by LDS-DMA.
@@ -119,7 +125,12 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
 #pragma unroll
     for (int k = 0; k < MS; k++)
       if (k < ns) {
-        __builtin_amdgcn_global_load_lds((const void *)(src + voff[k]), (lds_void *)(sb + k * ls), 16, 0, 0);
+        // a fresh 32-bit offset per load keeps the address in the form
+        // (scalar base + 32-bit lane offset) the load encodes directly,
+        // instead of a 64-bit vector add per load
+        unsigned o = voff[k];
+        asm volatile("" : "+v"(o));
+        __builtin_amdgcn_global_load_lds((const void *)(src + o), (lds_void *)(sb + k * ls), 16, 0, 0);
         n++;
       }
     if (q < N1) {
@@ -141,13 +152,12 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   // Per step, uniform in the workgroup: the y part of computeFeet for cell
   // y = q, and the update of the lane's x-interpolated E rows (buffer b).
   __device__ __forceinline__ void field_rows(int q, int b, double cy[3], double &yq) {
-    yq = g.minPhy[1] + q * g.dx[1];
-    const double xt1 = half_y(g, yq, vy, dt);
-    const int ip1 = __builtin_amdgcn_readfirstlane((int)(round(xt1) - 1));
-    const double py = xt1 - ip1;
-    cy[0] = .5 * (py - 1.) * (py - 2.);
-    cy[1] = -1. * (py) * (py - 2.);
-    cy[2] = .5 * (py) * (py - 1.);
+    const double *t = qtab + QT * q;
+    yq = t[0];
+    const int ip1 = __builtin_amdgcn_readfirstlane((int)t[1]);
+    cy[0] = t[2];
+    cy[1] = t[3];
+    cy[2] = t[4];
     const double2 *er = (const double2 *)(eb + b * 6 * EPR);
     if (ip1 == ip1p + 1) {
       gx[0] = gx[1];
@@ -176,6 +186,7 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
     OpenCell<A> &c = cell[R];
     c.acc = 0;
     c.key = -1;
+    c.kbit = 0;
     if (q >= N1) {
       // no cell: zero x-weights make its share of a common pass zero
 #pragma unroll
@@ -243,6 +254,7 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
     const bool fast = ok & ((unsigned)(ox + 1) < 2u) & (oy == oyw) & ((unsigned)(ovx + 1) < 2u) &
                       ((unsigned)(ovy + 1) < 2u) & !(force & 2);
     c.key = fast ? (ox + 1) | ((ovx + 1) << 1) | ((ovy + 1) << 2) : -1;
+    c.kbit = fast ? 1 << c.key : 0;
     // out of domain: the reference stops with "Time step is too large"
     bad |= !ok;
     if (ok & !fast) {
@@ -319,19 +331,14 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   // form with one cell per pass.
   template <int R> __device__ __forceinline__ void accumulate(int b) {
     const T *s = (const T *)((const char *)slab + b * slab_bytes);
-    int k0 = -1;
-    bool same = !(force & 1);
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const int k = cell[c].key;
-      if (k >= 0) {
-        if (k0 < 0) k0 = k;
-        else same = same && (k == k0);
-      }
-    }
+    // the distinct keys of the open cells as bits: one key (or none) when
+    // at most one bit is set
+    const int m = cell[0].kbit | cell[1].kbit | cell[2].kbit | cell[3].kbit;
+    const bool same = ((m & (m - 1)) == 0) & !(force & 1);
     A t[4];
     if (__builtin_amdgcn_ballot_w64(!same) == 0) {
-      pass_rows(s + row_base(k0 < 0 ? 0 : k0), t);
+      const int k0 = m ? __builtin_ctz(m) : 0;
+      pass_rows(s + row_base(k0), t);
       close_plane<R, 0>(t, 15u);
       close_plane<R, 1>(t, 15u);
       close_plane<R, 2>(t, 15u);
@@ -374,7 +381,7 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
     const int y = q - 3;
     if (y < 0 || y >= N1) return;
     const A v = cell[(R + 1) & 3].acc;
-    fout[ix + (long)N0 * (y + (long)N1 * (ivx + (long)N2 * ivy))] = (T)v;
+    fbase[(long)N0 * y] = (T)v;
   }
 
   // Step q: plane q sits in LDS buffer b = q mod NB; vector memory operations
@@ -388,8 +395,8 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   template <int R> __device__ __forceinline__ void step(int q, int b) {
     if (NB == 2 && q + 1 < nsteps) fetch(q + 1, b ^ 1);
     if (active) {
-      double cy[3], yq;
-      field_rows(q, b, cy, yq);
+      double cy[3] = {0., 0., 0.}, yq = 0.;
+      if (q < N1) field_rows(q, b, cy, yq);  // no cell opens beyond the last y
       open<R>(q, yq, cy);
       accumulate<R>(b);
       close<R>(q);
@@ -458,6 +465,7 @@ void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geo
   s.slab = (T *)ys_smem;
   s.eb = (double *)(ys_smem + NB * (size_t)s.slab_bytes);
   s.kconst = s.eb + NB * 6 * s.EPR;
+  s.qtab = s.kconst + s.KC_N;
   if (threadIdx.x == 0) {
     double *k = s.kconst;
 #pragma unroll
@@ -470,12 +478,27 @@ void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geo
     k[s.KC_VHI] = g.minPhy[2] + fc.rxwidth[2];
     k[s.KC_VHI + 1] = g.minPhy[3] + fc.rxwidth[3];
   }
-  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the table is written before the first barrier
   s.nsteps = s.N1 + 3;
   const int ntile = s.N2 / TVX;
   int tile, ivy;
   ys_block_tile(blockIdx.x, gridDim.x, ntile, s.N3, remap, tile, ivy);
   s.ivy = ivy;
+  {  // the y part of computeFeet for every cell y of this vy (VMO/Advection.hpp:174-196)
+    const double vyw = g.minPhy[3] + ivy * g.dx[3];
+    for (int q = threadIdx.x; q < s.N1; q += blockDim.x) {
+      const double yq = g.minPhy[1] + q * g.dx[1];
+      const double xt1 = half_y(g, yq, vyw, dt);
+      const int ip1 = (int)(round(xt1) - 1);
+      const double py = xt1 - ip1;
+      double *t = s.qtab + s.QT * q;
+      t[0] = yq;
+      t[1] = (double)ip1;
+      t[2] = .5 * (py - 1.) * (py - 2.);
+      t[3] = -1. * (py) * (py - 2.);
+      t[4] = .5 * (py) * (py - 1.);
+    }
+  }
+  __syncthreads();  // both tables are written before the first fetch reads them
   const int tid = threadIdx.x;
   s.active = tid < s.N0 * TVX;
   s.ix = s.active ? tid % s.N0 : 0;
@@ -485,6 +508,7 @@ void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geo
   s.vx = g.minPhy[2] + s.ivx * g.dx[2];
   s.vy = g.minPhy[3] + s.ivy * g.dx[3];
   s.oyw = s.vy > 0. ? -1 : 0;
+  s.fbase = fout + s.ix + (long)s.N0 * s.N1 * (s.ivx + (long)s.N2 * s.ivy);
   s.F0 = s.N0 + 6;
   s.S1 = s.N0 + 6;
   s.S2 = s.S1 * (s.N1 + 6);
@@ -521,6 +545,7 @@ void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geo
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     s.cell[c].key = -1;
+    s.cell[c].kbit = 0;
     s.cell[c].acc = 0;
 #pragma unroll
     for (int j = 0; j < 4; j++)
@@ -593,7 +618,7 @@ YsPlan ys_plan(const Geom &g, size_t tsz, int want_tvx = 0, int want_nb = 0) {
     const long ninstr = ys_slab_instr(N0, tvx, tsz);
     if ((ninstr + block / 64 - 1) / (block / 64) > 8) continue;  // DMA slots per wave
     const size_t lds = nb * (size_t)ninstr * 1024 + nb * 3 * (size_t)ys_epr(N0) * 2 * sizeof(double) +
-                       18 * sizeof(double);  // domain constants
+                       (18 + 5 * (size_t)N1) * sizeof(double);  // domain constants, per-step table
     if (lds > 160 * 1024) continue;
     p.tvx = tvx;
     p.nb = nb;

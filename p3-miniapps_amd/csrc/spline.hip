@@ -277,6 +277,7 @@ __global__ __launch_bounds__(XT == 256 ? 1024 : 512) void k_spline_tile(const T 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *tile = reinterpret_cast<T *>(smem_raw);  // [N+4][XT]
   constexpr int VPL = 16 / (int)sizeof(T), CPR = XT / VPL;  // 16-byte chunks per row
+  static_assert(CPR <= 64, "a tile row must fit one 64-lane DMA instruction");
   const int N = P * C;
   const bool comb = m.is1 == m.X0;
   const long XC = comb ? m.X0 * m.X1 : m.X0;  // contiguous input extent
@@ -410,7 +411,10 @@ static int tile_chunk(const LineMap &m, size_t tsz, int &P, int &XT, int mode = 
   // where they fit (float storage at N = 128: 135 KB of LDS, 1024 lanes)
   int wide = mode == SPLINE_TILE64 ? 0 : mode == SPLINE_TILE256 ? 2 : 1;
   if (const char *e = getenv("VLP4D_SPLINE_WIDE")) wide = atoi(e);
-  if (wide == 2 && m.N % 32 == 0 && (m.N / 32) * 256 <= 1024 && ((m.N / 32) & (m.N / 32 - 1)) == 0 &&
+  // (a tile row is moved by one DMA instruction of 64 lanes x 16 bytes, so a
+  // 256-position row needs 4-byte elements)
+  if (wide == 2 && tsz * 256 <= 1024 && m.N % 32 == 0 && (m.N / 32) * 256 <= 1024 &&
+      ((m.N / 32) & (m.N / 32 - 1)) == 0 &&
       (size_t)(m.N + 4) * 256 * tsz <= 160 * 1024 && (m.ils * tsz) % 16 == 0 && (m.is1 * tsz) % 16 == 0 &&
       (m.ios * tsz) % 16 == 0 && ((m.is1 == m.X0 ? m.X0 * m.X1 : m.X0) * (long)tsz) % 16 == 0) {
     P = m.N / 32;
@@ -450,7 +454,8 @@ template <typename T> void launch_spline_line(const T *in, T *out, const LineMap
       case 15: return launch_tile<T, 15, 64>(in, out, m, Pt, s);
       case 16: return launch_tile<T, 16, 64>(in, out, m, Pt, s);
       case 32:
-        if (XT == 256) return launch_tile<T, 32, 256>(in, out, m, Pt, s);
+        if constexpr (sizeof(T) * 256 <= 1024)  // float only (tile_chunk checks the same)
+          if (XT == 256) return launch_tile<T, 32, 256>(in, out, m, Pt, s);
         return launch_tile<T, 32, 128>(in, out, m, Pt, s);
       default: break;
     }
