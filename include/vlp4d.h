@@ -193,8 +193,9 @@ const char *vlp_timer_label(int which);
  * vx-blocked one), bits 8-11: its vx tile, bit 12: fused x/y spline + 2-D
  * advection kernel (replaces the separate x, y and advect2d kernels), bits
  * 13 / 14: the vx / vy spline pass uses 128-position LDS tiles, four lanes
- * per 128-point line).  No reference counterpart: introspection for tests
- * and reports. */
+ * per 128-point line, bit 15: both passes run as MFMA products with the line
+ * operator, VLP4D_SPLINE_MFMA=1 at creation; bits 13 and 14 are then clear).
+ * No reference counterpart: introspection for tests and reports. */
 int vlp_kernel_variants(const vlp_ctx *ctx);
 
 /* Message of the last error on this context (or the last global error when
@@ -202,7 +203,7 @@ int vlp_kernel_variants(const vlp_ctx *ctx);
 const char *vlp_last_error(const vlp_ctx *ctx);
 
 /* Autotuner: the 4-D advection's workgroup shape (vx tile, LDS plane
- * buffers) and the kernel of the vx and of the vy spline pass (128- or
+ * buffers) and the kernel of the vx and of the vy spline pass (256-, 128- or
  * 64-position LDS tiles, or the line kernels): every choice the geometry
  * admits is timed on the context's own buffers and the fastest is used by its
  * later steps; csv_path (optional) receives the timings, one row per choice
