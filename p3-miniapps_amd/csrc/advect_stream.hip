@@ -40,8 +40,9 @@ namespace {
 template <typename A> struct OpenCell {
   A w[4][4];
   A acc;
-  int key;
-  int kbit;  // 1 << key, or 0 without a streamed contribution
+  // 1 << key (key = (ox + 1) | (ovx + 1) << 1 | (ovy + 1) << 2, the stencil
+  // offsets of a streamed foot), or 0 without a streamed contribution
+  int kbit;
 };
 
 typedef __attribute__((address_space(3))) void lds_void;
@@ -139,7 +140,10 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
       for (int e = wave; e < 3 * cpr; e += nwave, n++) {
         const int k1 = (e >= cpr) + (e >= 2 * cpr), c = e - k1 * cpr;
         int row = ip1 + k1;
-        if (row < 0 || row >= N1) row = (row % N1 + N1) % N1;
+        // periodic wrap (row is wave-uniform; a scalar loop, no division
+        // constants kept live across the y loop)
+        while (row < 0) row += N1;
+        while (row >= N1) row -= N1;
         // points beyond N0 (last chunk) repeat the row's last point into the padding
         const int pt = min(64 * c + lane, N0 - 1);
         __builtin_amdgcn_global_load_lds((const void *)(exy + 2 * ((long)row * N0 + pt)),
@@ -180,12 +184,11 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
 
   // Open cell y = q in slot R: the rest of computeFeet (VMO/Advection.hpp:
   // 166-218), the bound check, B-spline weights and stencil offsets.  A cell
-  // that takes no streamed contribution gets key -1; accumulate() then never
-  // adds to it, whatever its weights hold.
+  // that takes no streamed contribution gets no key bit and zero x-weights,
+  // so that a common pass adds nothing to it.
   template <int R> __device__ __forceinline__ void open(int q, double yq, const double cy[3]) {
     OpenCell<A> &c = cell[R];
     c.acc = 0;
-    c.key = -1;
     c.kbit = 0;
     if (q >= N1) {
       // no cell: zero x-weights make its share of a common pass zero
@@ -229,16 +232,11 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
     double xs[4];
     xs[0] = x - dt * xt2;
     xs[1] = yq - dt * xt3;
-    {
-      const double a = vx - dt * ftmp1, hi = vhi[0], lo = minp[2];
-      const double m = (hi < a) ? hi : a;
-      xs[2] = (m < lo) ? lo : m;
-    }
-    {
-      const double a = vy - dt * ftmp2, hi = vhi[1], lo = minp[3];
-      const double m = (hi < a) ? hi : a;
-      xs[3] = (m < lo) ? lo : m;
-    }
+    // the velocity clamp max(min(v*, vmax), vmin) as v_min / v_max: they
+    // differ from the comparisons only for a NaN v*, which makes x* NaN
+    // too, so the bound check below still fails
+    xs[2] = fmax(fmin(vx - dt * ftmp1, vhi[0]), minp[2]);
+    xs[3] = fmax(fmin(vy - dt * ftmp2, vhi[1]), minp[3]);
     // all eight comparisons, no short cut (NaN fails them)
     bool ok = true;
 #pragma unroll
@@ -247,14 +245,15 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const double fj = invd[j] * (xs[j] - minp[j]);
-      ip[j] = ok ? (int)floor(fj) : 0;
-      bspline_eta<A>((A)(fj - ip[j]), c.w[j]);
+      const double fl = floor(fj);
+      ip[j] = ok ? (int)fl : 0;
+      // fj - floor(fj) is fj - ip exactly wherever the foot is valid
+      bspline_eta<A>((A)(fj - fl), c.w[j]);
     }
     const int ox = ip[0] - ix, oy = ip[1] - q, ovx = ip[2] - ivx, ovy = ip[3] - ivy;
     const bool fast = ok & ((unsigned)(ox + 1) < 2u) & (oy == oyw) & ((unsigned)(ovx + 1) < 2u) &
                       ((unsigned)(ovy + 1) < 2u) & !(force & 2);
-    c.key = fast ? (ox + 1) | ((ovx + 1) << 1) | ((ovy + 1) << 2) : -1;
-    c.kbit = fast ? 1 << c.key : 0;
+    c.kbit = fast ? 1 << ((ox + 1) | ((ovx + 1) << 1) | ((ovy + 1) << 2)) : 0;
     // out of domain: the reference stops with "Time step is too large"
     bad |= !ok;
     if (ok & !fast) {
@@ -266,7 +265,12 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
       for (int j = 0; j < 4; j++)
 #pragma unroll
         for (int k = 0; k < 4; k++) e[j][k] = c.w[j][k];
-      c.acc = gather4d<T, A>(c4, S1, S2, S3, ip, e);
+      // the C4 strides are rebuilt here, on this rare path, rather than
+      // kept in scalar registers across the loop
+      int n0 = N0, n1 = N1, n2 = N2;
+      asm volatile("" : "+s"(n0), "+s"(n1), "+s"(n2));
+      const long s1 = n0 + 6, s2 = s1 * (n1 + 6), s3 = s2 * (n2 + 6);
+      c.acc = gather4d<T, A>(c4, s1, s2, s3, ip, e);
 #pragma unroll
       for (int k = 0; k < 4; k++) c.w[0][k] = 0;  // no share of a common pass
     }
@@ -347,18 +351,18 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
     }
     unsigned todo = 0;
 #pragma unroll
-    for (int c = 0; c < 4; c++) todo |= (cell[c].key >= 0 ? 1u : 0u) << c;
+    for (int c = 0; c < 4; c++) todo |= (cell[c].kbit ? 1u : 0u) << c;
     while (__builtin_amdgcn_ballot_w64(todo != 0)) {
       int kp = 0;
 #pragma unroll
       for (int c = 3; c >= 0; c--)
-        if ((todo >> c) & 1) kp = cell[c].key;
+        if ((todo >> c) & 1) kp = cell[c].kbit;
       unsigned sel = 0;
 #pragma unroll
       for (int c = 0; c < 4; c++)
-        if (((todo >> c) & 1) && cell[c].key == kp && !((force & 1) && sel)) sel |= 1u << c;
+        if (((todo >> c) & 1) && cell[c].kbit == kp && !((force & 1) && sel)) sel |= 1u << c;
       todo &= ~sel;
-      pass_rows(s + row_base(kp), t);
+      pass_rows(s + row_base(kp ? __builtin_ctz(kp) : 0), t);
       close_plane<R, 0>(t, sel);
       close_plane<R, 1>(t, sel);
       close_plane<R, 2>(t, sel);
@@ -544,7 +548,6 @@ void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geo
   s.cbase = 1 + s.F0 * (s.N1 + 6) * ((tile * TVX + 1) + (s.N2 + 6) * (s.ivy + 1));
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    s.cell[c].key = -1;
     s.cell[c].kbit = 0;
     s.cell[c].acc = 0;
 #pragma unroll
