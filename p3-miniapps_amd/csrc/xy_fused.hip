@@ -140,45 +140,96 @@ __global__ __launch_bounds__(XY_NT) void k_xy_adv2d(const T *__restrict__ f, T *
       const T *pc = pl + (xbad ? 0 : jx0 + 1);  // stencil column jx0-1 at LDS column jx0+1
       A gw[4];  // x-interpolated rows jw .. jw+3 (LDS rows)
       int jw = -1000;
+      constexpr int GP = sizeof(A) == sizeof(float) ? 4 : 2;
+      static_assert(XY_NPT % GP == 0, "groups tile the point range");
+      // Points go in groups of GP consecutive y (four for float arithmetic,
+      // two for double, whose tables take twice the registers).  The
+      // segment's lanes share y, so jy0 and the group's path are uniform in a
+      // wave when N0 is a multiple of 64.  When the GP stencils each move down
+      // by one row from the previous point's (the usual case), the group's
+      // tables and its GP new rows are all read before the arithmetic (one
+      // LDS round trip per group instead of several per point); otherwise the
+      // points go one at a time.
 #pragma unroll
-      for (int k = 0; k < XY_NPT; k++) {
-        const int y = y0 + k;
-        if (k < ylen && y < N1) {
-          const int jy0 = jty[y];
-          A etay[4];
+      for (int k0 = 0; k0 < XY_NPT; k0 += GP) {
+        const int yg = y0 + k0;
+        if (k0 < ylen && yg < N1) {
+          int jr[GP];
+          A ey[GP][4];
 #pragma unroll
-          for (int jy = 0; jy < 4; jy++) etay[jy] = ety[4 * y + jy];
-          A val = 0;
-          if (xbad || jy0 < -1 || jy0 > N1 - 1) {
-            bad = 1;  // interp_2D returns 1 and leaves 0 (VMO/Advection.hpp:148-151)
-          } else {
-            const int jr = jy0 + 1;  // stencil row jy0-1 at LDS row jy0+1
-            if (jr == jw + 1) {
+          for (int j = 0; j < GP; j++) {
+            const int y = min(yg + j, N1 - 1);
+            jr[j] = jty[y] + 1;  // stencil row jy0-1 at LDS row jy0+1
+#pragma unroll
+            for (int jy = 0; jy < 4; jy++) ey[j][jy] = ety[4 * y + jy];
+          }
+          const bool full = k0 + GP - 1 < ylen && yg + GP - 1 < N1;
+          bool slide = full && !xbad && jr[0] == jw + 1;
+#pragma unroll
+          for (int j = 0; j < GP; j++) slide = slide && jr[j] >= 0 && jr[j] <= N1 && jr[j] == jr[0] + j;
+          if (slide) {
+            T nr[GP][4];  // rows jw+4 .. jw+3+GP
+#pragma unroll
+            for (int j = 0; j < GP; j++) {
+              const T *p = pc + (jw + 4 + j) * R;
+#pragma unroll
+              for (int jx = 0; jx < 4; jx++) nr[j][jx] = p[jx];
+            }
+#pragma unroll
+            for (int j = 0; j < GP; j++) {
               gw[0] = gw[1];
               gw[1] = gw[2];
               gw[2] = gw[3];
-              const T *p = pc + (jr + 3) * R;
               A s = 0;
 #pragma unroll
-              for (int jx = 0; jx < 4; jx++) s += etax[jx] * (A)p[jx];
+              for (int jx = 0; jx < 4; jx++) s += etax[jx] * (A)nr[j][jx];
               gw[3] = s;
-            } else if (jr != jw) {
+              A val = 0;
 #pragma unroll
-              for (int jy = 0; jy < 4; jy++) {
-                const T *p = pc + (jr + jy) * R;
-                A s = 0;
+              for (int jy = 0; jy < 4; jy++) val += ey[j][jy] * gw[jy];
+              rho[k0 + j] += (double)val;
+            }
+            jw = jr[GP - 1];
+          } else {
 #pragma unroll
-                for (int jx = 0; jx < 4; jx++) s += etax[jx] * (A)p[jx];
-                gw[jy] = s;
+            for (int j = 0; j < GP; j++) {
+              const int k = k0 + j, y = yg + j;
+              if (k < ylen && y < N1) {
+                const int jy0 = jr[j] - 1;
+                A val = 0;
+                if (xbad || jy0 < -1 || jy0 > N1 - 1) {
+                  bad = 1;  // interp_2D returns 1 and leaves 0 (VMO/Advection.hpp:148-151)
+                } else {
+                  const int r = jr[j];
+                  if (r == jw + 1) {
+                    gw[0] = gw[1];
+                    gw[1] = gw[2];
+                    gw[2] = gw[3];
+                    const T *p = pc + (r + 3) * R;
+                    A s = 0;
+#pragma unroll
+                    for (int jx = 0; jx < 4; jx++) s += etax[jx] * (A)p[jx];
+                    gw[3] = s;
+                  } else if (r != jw) {
+#pragma unroll
+                    for (int jy = 0; jy < 4; jy++) {
+                      const T *p = pc + (r + jy) * R;
+                      A s = 0;
+#pragma unroll
+                      for (int jx = 0; jx < 4; jx++) s += etax[jx] * (A)p[jx];
+                      gw[jy] = s;
+                    }
+                  }
+                  jw = r;
+#pragma unroll
+                  for (int jy = 0; jy < 4; jy++) val += ey[j][jy] * gw[jy];
+                }
+                rho[k] += (double)val;
               }
             }
-            jw = jr;
-#pragma unroll
-            for (int jy = 0; jy < 4; jy++) val += etay[jy] * gw[jy];
           }
-          rho[k] += (double)val;
         }
-        __builtin_amdgcn_sched_barrier(0);  // one point at a time (register pressure)
+        __builtin_amdgcn_sched_barrier(0);  // one group at a time (register pressure)
       }
     }
     __syncthreads();
