@@ -66,6 +66,7 @@ struct vlp_ctx {
   double *adj_c4s = nullptr, *adj_blk = nullptr, *adj_tmp = nullptr, *adj_cxy = nullptr;
   double *adj_E = nullptr, *adj_rho = nullptr, *adj_k = nullptr, *adj_g = nullptr, *adj_grad = nullptr;
   double *adj_Mt[4] = {};  // M_d^T of the line spline operator, N_d x (N_d + 4)
+  double *adj_M[4] = {};   // M_d itself, (N_d + 4) x N_d row-major (the tiled contraction's operand)
 };
 
 namespace {
@@ -144,7 +145,8 @@ void free_all(vlp_ctx *c) {
                   c->fb.rho,   c->fb.ex,    c->fb.ey,     c->fb.exy, c->fb.spec1, c->fb.spec2, c->fb.twx,  c->fb.twy,
                   c->fb.diag_part, c->fb.rho_tmp, c->fb.diag, c->d_err, c->adj_c4s, c->adj_blk, c->adj_tmp,
                   c->adj_cxy, c->adj_E, c->adj_rho, c->adj_k, c->adj_g, c->adj_grad, c->adj_Mt[0], c->adj_Mt[1],
-                  c->adj_Mt[2], c->adj_Mt[3], c->lag_tab, c->mf_m[0], c->mf_m[1]};
+                  c->adj_Mt[2], c->adj_Mt[3], c->adj_M[0], c->adj_M[1], c->adj_M[2], c->adj_M[3], c->lag_tab,
+                  c->mf_m[0], c->mf_m[1]};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_err) (void)hipHostFree(c->h_err);
@@ -1049,6 +1051,7 @@ static int adjoint_prepare(vlp_ctx *c) {
   ADJ_ALLOC(c->adj_grad, c->n_f * sizeof(double));
   const bool first_mt = c->adj_Mt[0] == nullptr;
   for (int d = 0; d < 4; d++) ADJ_ALLOC(c->adj_Mt[d], (size_t)g.n[d] * (g.n[d] + 4) * sizeof(double));
+  for (int d = 0; d < 4; d++) ADJ_ALLOC(c->adj_M[d], (size_t)g.n[d] * (g.n[d] + 4) * sizeof(double));
 #undef ADJ_ALLOC
   if (first_mt) {
     // M_d from the product spline kernel on unit vectors: line j = e_j ->
@@ -1069,6 +1072,12 @@ static int adjoint_prepare(vlp_ctx *c) {
       const hipError_t e = hipStreamSynchronize(s);
       (void)hipFree(d_id);
       if (e != hipSuccess) return set_err(c, VLP_E_HIP, "line operator");
+      // and M_d row-major, transposed on the host
+      std::vector<double> mt((size_t)N * (N + 4)), mm(mt.size());
+      HIP_TRY(c, hipMemcpy(mt.data(), c->adj_Mt[d], mt.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (int j = 0; j < N; j++)
+        for (int q = 0; q < N + 4; q++) mm[(size_t)q * N + j] = mt[(size_t)j * (N + 4) + q];
+      HIP_TRY(c, hipMemcpy(c->adj_M[d], mm.data(), mm.size() * sizeof(double), hipMemcpyHostToDevice));
     }
   }
   // idcases 2 and 6 set an analytic E that does not depend on f (VMO/Field.cpp:81-108)
@@ -1091,6 +1100,12 @@ static int adjoint_prepare(vlp_ctx *c) {
   return VLP_OK;
 }
 
+// VLP4D_ADJ_TILED=0: the per-output line contraction instead of the LDS tiles.
+static bool adj_tiled() {
+  if (const char *e = getenv("VLP4D_ADJ_TILED")) return atoi(e) != 0;
+  return true;
+}
+
 // (d f^{n+1}/d f^n)^T adj_g -> adj_grad at the context's f, on its stream.
 static int adjoint_core(vlp_ctx *c) {
   const Geom &g = c->g;
@@ -1110,17 +1125,24 @@ static int adjoint_core(vlp_ctx *c) {
                            field_term, s);
   // direct term: (M_x^T M_y^T M_vx^T M_vy^T) of the coefficient adjoint
   vlp::launch_c4_block(g, c->adj_c4s, c->adj_blk, s);
-  vlp::launch_axis_contract(c->adj_Mt[3], (int)E3, (int)N3, c->adj_blk, c->adj_tmp, 1, E0 * E1 * E2, false, s);
-  vlp::launch_axis_contract(c->adj_Mt[2], (int)E2, (int)N2, c->adj_tmp, c->adj_blk, N3, E0 * E1, false, s);
-  vlp::launch_axis_contract(c->adj_Mt[1], (int)E1, (int)N1, c->adj_blk, c->adj_tmp, N2 * N3, E0, false, s);
-  vlp::launch_axis_contract(c->adj_Mt[0], (int)E0, (int)N0, c->adj_tmp, c->adj_grad, N1 * N2 * N3, 1, false, s);
+  const bool tiled = adj_tiled();
+  vlp::launch_axis_contract(c->adj_Mt[3], (int)E3, (int)N3, c->adj_blk, c->adj_tmp, 1, E0 * E1 * E2, false, s,
+                            tiled ? c->adj_M[3] : nullptr);
+  vlp::launch_axis_contract(c->adj_Mt[2], (int)E2, (int)N2, c->adj_tmp, c->adj_blk, N3, E0 * E1, false, s,
+                            tiled ? c->adj_M[2] : nullptr);
+  vlp::launch_axis_contract(c->adj_Mt[1], (int)E1, (int)N1, c->adj_blk, c->adj_tmp, N2 * N3, E0, false, s,
+                            tiled ? c->adj_M[1] : nullptr);
+  vlp::launch_axis_contract(c->adj_Mt[0], (int)E0, (int)N0, c->adj_tmp, c->adj_grad, N1 * N2 * N3, 1, false, s,
+                            tiled ? c->adj_M[0] : nullptr);
   if (field_term) {
     // through E: K^T, then the half step and C_xy^T
     vlp::launch_poisson_T(c->adj_E, c->adj_k, c->adj_k + N0 * N1, c->adj_rho, (int)N0, (int)N1, s);
     HIP_TRY(c, hipMemsetAsync(c->adj_cxy, 0, c->n_cxy * sizeof(double), s));
     vlp::launch_adj_advect2d(c->adj_rho, g, 0.5 * c->cfg.dt, c->adj_cxy, s);
-    vlp::launch_axis_contract(c->adj_Mt[1], (int)E1, (int)N1, c->adj_cxy, c->adj_tmp, N2 * N3, E0, false, s);
-    vlp::launch_axis_contract(c->adj_Mt[0], (int)E0, (int)N0, c->adj_tmp, c->adj_grad, N1 * N2 * N3, 1, true, s);
+    vlp::launch_axis_contract(c->adj_Mt[1], (int)E1, (int)N1, c->adj_cxy, c->adj_tmp, N2 * N3, E0, false, s,
+                              tiled ? c->adj_M[1] : nullptr);
+    vlp::launch_axis_contract(c->adj_Mt[0], (int)E0, (int)N0, c->adj_tmp, c->adj_grad, N1 * N2 * N3, 1, true, s,
+                              tiled ? c->adj_M[0] : nullptr);
   }
   return check_launch(c, "adjoint");
 }

@@ -139,8 +139,10 @@ bool launch_lag_xy(const Geom &g, const double *in, double *out, const void *tab
 
 void launch_adj_advect4d(const Geom &g, const double *c4, const double *ex, const double *ey, double dt,
                          const double *gin, double *lc4, double *lE, bool field_term, hipStream_t s);
+// out[o][n][i] (+)= sum_m Mt[n][m] in[o][m][i]; with M (= Mt^T, Min x Nout
+// row-major) given, by LDS tiles of 64 columns
 void launch_axis_contract(const double *Mt, int Min, int Nout, const double *in, double *out, long O, long I,
-                          bool accumulate, hipStream_t s);
+                          bool accumulate, hipStream_t s, const double *M = nullptr);
 void launch_c4_block(const Geom &g, const double *c4s, double *blk, hipStream_t s);
 void launch_poisson_T(const double *lE, const double *kx, const double *ky, double *lrho, int N0, int N1,
                       hipStream_t s);
