@@ -110,31 +110,40 @@ __global__ __launch_bounds__(256) void k_adj_advect4d(const double *__restrict__
     }
 }
 
-// The same scatter by blocks of cells.  A workgroup takes BX x 2 x 2 x 2
-// cells (x fastest, one cell per lane) and first adds their contributions
-// into LDS copies of the coefficient region they reach when every foot lies
-// within one cell below its own (ip_d - c_d in {-1, 0}: positions c - 2 ..
-// c + 2 per dimension) and of the matching E region; a contribution outside
-// them goes straight to global memory.  The regions are then added to lc4
-// and lE, one atomic per nonzero entry instead of one per cell and stencil
-// point: (BX + 4) 6^3 / (8 BX) ~ 29 global atomics per cell instead of 256
-// (+ 18 for E).  The sums agree with the per-cell scatter to round-off (the
-// order of the adds differs, as it does between runs of either form).
+// Fixed-point form of a coefficient-adjoint contribution: x * S rounded to
+// the nearest integer (|x S| < 2^51: adding 1.5 * 2^52 leaves the integer in
+// the low mantissa bits).  Integer sums do not depend on the order of the
+// adds, so the result is the same on every run.
+__device__ __forceinline__ unsigned long long to_fixed(double x) {
+  const double M = 6755399441055744.0;  // 1.5 * 2^52
+  return (unsigned long long)(__double_as_longlong(x + M) - __double_as_longlong(M));
+}
+
+// The same scatter by blocks of cells, with reproducible sums.  A workgroup
+// takes BX x 2 x 2 x 2 cells (x fastest, one cell per lane) and adds their
+// coefficient-adjoint contributions, as 64-bit fixed-point integers (scale
+// S = *scl, a power of two with |g| S < 2^50), into an LDS copy of the
+// coefficient region they reach when every foot lies within one cell below
+// its own (ip_d - c_d in {-1, 0}: positions c - 2 .. c + 2 per dimension);
+// a contribution outside it goes straight to lc4 (read as 64-bit integers).
+// The region is then added to lc4, one atomic per nonzero entry instead of
+// one per cell and stencil point: (BX + 4) 6^3 / (8 BX) ~ 29 global atomics
+// per cell instead of 256.  With the field term, each cell's pair
+// (ax, ay) -- g times the derivative of the interpolation along the foot,
+// times d foot / d E-bar -- is written to axy, for the E gathers below.
 template <int BX>
 __global__ __launch_bounds__(8 * BX) void k_adj_advect4d_blk(const double *__restrict__ c4,
                                                            const double *__restrict__ ex,
                                                            const double *__restrict__ ey, Geom g, FeetConsts fc,
                                                            double dt, const double *__restrict__ gin,
-                                                           double *__restrict__ lc4, double *__restrict__ lE,
+                                                           unsigned long long *__restrict__ lc4,
+                                                           double *__restrict__ axy, const double *__restrict__ scl,
                                                            int field_term) {
-  constexpr int RX = BX + 4, RY = 6, RV = 6;          // coefficient region (x, y, vx, vy)
+  constexpr int RX = BX + 4, RY = 6, RV = 6;  // coefficient region (x, y, vx, vy)
   constexpr int NREG = RX * RY * RV * RV;
-  constexpr int EXR = BX + 8, EYR = 10;                // E region, (Ex, Ey) pairs
-  constexpr int NER = 2 * EXR * EYR;
   constexpr int NT = 8 * BX;
   extern __shared__ __align__(16) unsigned char adj_smem[];
-  double *reg = (double *)adj_smem;
-  double *ereg = reg + NREG;
+  unsigned long long *reg = (unsigned long long *)adj_smem;
   const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
   const int nbx = (N0 + BX - 1) / BX, nby = (N1 + 1) / 2, nbv = (N2 + 1) / 2;
   long b = blockIdx.x;
@@ -145,8 +154,8 @@ __global__ __launch_bounds__(8 * BX) void k_adj_advect4d_blk(const double *__res
   const int bvx = (int)(b % nbv), bvy = (int)(b / nbv);
   const int x0 = bx * BX, y0 = 2 * by, vx0 = 2 * bvx, vy0 = 2 * bvy;
   const int tid = threadIdx.x;
-  for (int i = tid; i < NREG; i += NT) reg[i] = 0.;
-  for (int i = tid; i < NER; i += NT) ereg[i] = 0.;
+  const double S = scl[0];
+  for (int i = tid; i < NREG; i += NT) reg[i] = 0ull;
   __syncthreads();
   const int lx = tid % BX, r = tid / BX;
   const int ix = x0 + lx, iy = y0 + (r & 1), ivx = vx0 + ((r >> 1) & 1), ivy = vy0 + (r >> 2);
@@ -160,6 +169,7 @@ __global__ __launch_bounds__(8 * BX) void k_adj_advect4d_blk(const double *__res
     const int ip1 = (int)(round(xt1) - 1);
     EGlobal E{ex, ey, N0, N1, ip1};
     double xs[4];
+    double ax = 0., ay = 0.;
     // (a zero cotangent or a foot out of the box adds nothing; the forward
     // step reports the latter)
     if (gv != 0. && compute_foot(g, fc, E, xt1, ip1, x, y, vx, vy, dt, xs)) {
@@ -184,7 +194,7 @@ __global__ __launch_bounds__(8 * BX) void k_adj_advect4d_blk(const double *__res
       double dI[4] = {0., 0., 0., 0.};
       double gw0[4];
 #pragma unroll
-      for (int k0 = 0; k0 < 4; k0++) gw0[k0] = gv * w[0][k0];
+      for (int k0 = 0; k0 < 4; k0++) gw0[k0] = gv * S * w[0][k0];
       for (int k3 = 0; k3 < 4; k3++) {
         double v = 0., vd0 = 0., vd1 = 0., vd2 = 0.;
         for (int k2 = 0; k2 < 4; k2++) {
@@ -197,9 +207,9 @@ __global__ __launch_bounds__(8 * BX) void k_adj_advect4d_blk(const double *__res
             const long orow = base + S1 * k1 + S2 * k2 + S3 * k3;
 #pragma unroll
             for (int k0 = 0; k0 < 4; k0++) {
-              const double val = gw0[k0] * w123;
-              if (inreg) atomicAdd(reg + lrow * RX + r0 + k0, val);
-              else atomicAdd(lc4 + orow + k0, val);
+              const unsigned long long q = to_fixed(gw0[k0] * w123);
+              if (inreg) atomicAdd(reg + lrow * RX + r0 + k0, q);
+              else atomicAdd(lc4 + orow + k0, q);
             }
             if (field_term) {
               double s0 = 0., sd = 0.;
@@ -226,57 +236,125 @@ __global__ __launch_bounds__(8 * BX) void k_adj_advect4d_blk(const double *__res
       }
       if (field_term) {
         const double vhi2 = g.minPhy[2] + fc.rxwidth[2], vhi3 = g.minPhy[3] + fc.rxwidth[3];
-        const double ax = gv * (dI[0] * g.inv_dx[0] * 0.5 * dt * dt +
-                                ((xs[2] > g.minPhy[2] && xs[2] < vhi2) ? -dt * dI[2] * g.inv_dx[2] : 0.));
-        const double ay = gv * (dI[1] * g.inv_dx[1] * 0.5 * dt * dt +
-                                ((xs[3] > g.minPhy[3] && xs[3] < vhi3) ? -dt * dI[3] * g.inv_dx[3] : 0.));
-        double xt0 = x - 0.5 * dt * vx;
-        xt0 = g.inv_dx[0] * (xt0 - g.minPhy[0]);
-        const int ip0 = (int)(round(xt0) - 1);
-        const double px = xt0 - ip0, py = xt1 - ip1;
-        const double cx[3] = {.5 * (px - 1.) * (px - 2.), -1. * (px) * (px - 2.), .5 * (px) * (px - 1.)};
-        const double cy[3] = {.5 * (py - 1.) * (py - 2.), -1. * (py) * (py - 2.), .5 * (py) * (py - 1.)};
-        // E nodes ip0 .. ip0+2 (x), ip1 .. ip1+2 (y), unwrapped, against
-        // the E region's origin (x0 - 4, y0 - 4)
-        const int e0 = ip0 - (x0 - 4), e1 = ip1 - (y0 - 4);
-        const bool ine = e0 >= 0 && e0 + 2 < EXR && e1 >= 0 && e1 + 2 < EYR;
-        for (int k1 = 0; k1 < 3; k1++)
-          for (int k0 = 0; k0 < 3; k0++) {
-            const double cc = cx[k0] * cy[k1];
-            if (ine) {
-              double *q = ereg + 2 * ((e1 + k1) * EXR + e0 + k0);
-              atomicAdd(q, cc * ax);
-              atomicAdd(q + 1, cc * ay);
-            } else {
-              const int jx = ((ip0 + k0) % N0 + N0) % N0, jy = ((ip1 + k1) % N1 + N1) % N1;
-              const long e = 2 * ((long)jy * N0 + jx);
-              atomicAdd(lE + e, cc * ax);
-              atomicAdd(lE + e + 1, cc * ay);
-            }
-          }
+        ax = gv * (dI[0] * g.inv_dx[0] * 0.5 * dt * dt +
+                   ((xs[2] > g.minPhy[2] && xs[2] < vhi2) ? -dt * dI[2] * g.inv_dx[2] : 0.));
+        ay = gv * (dI[1] * g.inv_dx[1] * 0.5 * dt * dt +
+                   ((xs[3] > g.minPhy[3] && xs[3] < vhi3) ? -dt * dI[3] * g.inv_dx[3] : 0.));
       }
+    }
+    if (field_term) {
+      axy[2 * t] = ax;
+      axy[2 * t + 1] = ay;
     }
   }
   __syncthreads();
-  // the regions to global memory: region position (r0, r1, r2, r3) is the
+  // the region to global memory: region position (r0, r1, r2, r3) is the
   // coefficient at position (x0 - 2 + r0, ...), storage index + 3 per
   // dimension; only entries some cell reached are nonzero
   for (int i = tid; i < NREG; i += NT) {
-    const double v = reg[i];
-    if (v != 0.) {
+    const unsigned long long v = reg[i];
+    if (v != 0ull) {
       const int q0 = i % RX, q1 = (i / RX) % RY, q2 = (i / (RX * RY)) % RV, q3 = i / (RX * RY * RV);
       const long o = (x0 + 1 + q0) + S1 * (y0 + 1 + q1) + S2 * (vx0 + 1 + q2) + S3 * (long)(vy0 + 1 + q3);
       atomicAdd(lc4 + o, v);
     }
   }
-  for (int i = tid; i < NER; i += NT) {
-    const double v = ereg[i];
-    if (v != 0.) {
-      const int p = i >> 1, c = i & 1, q0 = p % EXR, q1 = p / EXR;
-      const int jx = ((x0 - 4 + q0) % N0 + N0) % N0, jy = ((y0 - 4 + q1) % N1 + N1) % N1;
-      atomicAdd(lE + 2 * ((long)jy * N0 + jx) + c, v);
+}
+
+// Scale of the fixed-point coefficient adjoint: the largest |g| (as the bit
+// pattern of a non-negative double, whose order is the numeric order), then
+// S = 2^(50 - e) with max|g| < 2^e, and 1/S.
+__global__ __launch_bounds__(256) void k_absmax_bits(const double *__restrict__ gin, long n,
+                                                     unsigned long long *__restrict__ mx) {
+  unsigned long long m = 0ull;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(gin[t]));
+    m = b > m ? b : m;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_down(m, off, 64);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(mx, m);
+}
+__global__ void k_fixed_zero(unsigned long long *__restrict__ mx) {
+  if (threadIdx.x == 0) *mx = 0ull;
+}
+__global__ void k_fixed_scale(const unsigned long long *__restrict__ mx, double *__restrict__ scl) {
+  if (threadIdx.x != 0) return;
+  const double m = __longlong_as_double((long long)*mx);
+  double S = 1.;
+  if (m > 0. && m <= 1.7976931348623157e308) {
+    int e = 0;
+    (void)frexp(m, &e);  // m < 2^e
+    S = ldexp(1., 50 - e);
+  }
+  scl[0] = S;
+  scl[1] = 1. / S;
+}
+
+// E adjoint by gathers, in a fixed order.  computeFeet takes E-bar from the
+// 3 x 3 nodes (ip0 + k0, ip1 + k1) (mod N0, N1) with weights cx[k0] cy[k1],
+// ip0 and cx depending on (x, vx) and ip1 and cy on (y, vy).  First, for
+// every (jx, y, vy): the sum over vx and over the cells x whose node
+// ip0 + k0 is jx of cx[k0] (ax, ay)(x, y, vx, vy) -- ip0 = x + round(d) - 1
+// with d = -dt vx / (2 dx) up to rounding, so x is one of five candidates,
+// each checked exactly; then for every node (jx, jy) the same over (y, vy)
+// with cy.
+__global__ __launch_bounds__(256) void k_adjE_x(const double *__restrict__ axy, Geom g, double dt,
+                                                double *__restrict__ part) {
+  const int N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
+  const long n = (long)N0 * N1 * N3;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int jx = (int)(t % N0), iy = (int)((t / N0) % N1), ivy = (int)(t / ((long)N0 * N1));
+  double sx = 0., sy = 0.;
+  for (int ivx = 0; ivx < N2; ivx++) {
+    const double vx = g.minPhy[2] + ivx * g.dx[2];
+    const int sh = (int)round(-0.5 * dt * vx * g.inv_dx[0]) - 1;
+    for (int q = -3; q <= 1; q++) {
+      const int ix = ((jx - sh + q) % N0 + N0) % N0;
+      double xt0 = (g.minPhy[0] + ix * g.dx[0]) - 0.5 * dt * vx;
+      xt0 = g.inv_dx[0] * (xt0 - g.minPhy[0]);
+      const int ip0 = (int)(round(xt0) - 1);
+      const int k0 = ((jx - ip0) % N0 + N0) % N0;
+      if (k0 > 2) continue;
+      const double px = xt0 - ip0;
+      const double cx = k0 == 0 ? .5 * (px - 1.) * (px - 2.) : k0 == 1 ? -1. * (px) * (px - 2.) : .5 * (px) * (px - 1.);
+      const long c = ix + (long)N0 * (iy + (long)N1 * (ivx + (long)N2 * ivy));
+      sx += cx * axy[2 * c];
+      sy += cx * axy[2 * c + 1];
     }
   }
+  part[2 * t] = sx;
+  part[2 * t + 1] = sy;
+}
+__global__ __launch_bounds__(256) void k_adjE_y(const double *__restrict__ part, Geom g, double dt,
+                                                double *__restrict__ lE) {
+  const int N0 = g.n[0], N1 = g.n[1], N3 = g.n[3];
+  const long n = (long)N0 * N1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int jx = (int)(t % N0), jy = (int)(t / N0);
+  double sx = 0., sy = 0.;
+  for (int ivy = 0; ivy < N3; ivy++) {
+    const double vy = g.minPhy[3] + ivy * g.dx[3];
+    const int sh = (int)round(-0.5 * dt * vy * g.inv_dx[1]) - 1;
+    for (int q = -3; q <= 1; q++) {
+      const int iy = ((jy - sh + q) % N1 + N1) % N1;
+      const double xt1 = half_y(g, g.minPhy[1] + iy * g.dx[1], vy, dt);
+      const int ip1 = (int)(round(xt1) - 1);
+      const int k1 = ((jy - ip1) % N1 + N1) % N1;
+      if (k1 > 2) continue;
+      const double py = xt1 - ip1;
+      const double cy = k1 == 0 ? .5 * (py - 1.) * (py - 2.) : k1 == 1 ? -1. * (py) * (py - 2.) : .5 * (py) * (py - 1.);
+      const long pidx = jx + (long)N0 * (iy + (long)N1 * ivy);
+      sx += cy * part[2 * pidx];
+      sy += cy * part[2 * pidx + 1];
+    }
+  }
+  lE[2 * t] = sx;
+  lE[2 * t + 1] = sy;
 }
 
 // out[o][n][i] (+)= sum_m Mt[n][m] in[o][m][i]: a line operator's transpose
@@ -365,14 +443,18 @@ __global__ __launch_bounds__(256) void k_axis_contract_t(const double *__restric
 }
 
 // Contiguous copy of the coefficient block (positions -2..N+1 per dimension)
-// out of the C4 storage layout (positions -3..N+2).
-__global__ __launch_bounds__(256) void k_c4_block(const double *__restrict__ c4s, double *__restrict__ blk, Geom g) {
+// out of the C4 storage layout (positions -3..N+2); with scl, the storage
+// holds 64-bit fixed-point sums, converted with the factor scl[1] = 1/S.
+__global__ __launch_bounds__(256) void k_c4_block(const double *__restrict__ c4s, double *__restrict__ blk, Geom g,
+                                                  const double *__restrict__ scl) {
   const long E0 = g.n[0] + 4, E1 = g.n[1] + 4, E2 = g.n[2] + 4, E3 = g.n[3] + 4;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= E0 * E1 * E2 * E3) return;
   const long a = t % E0, b = (t / E0) % E1, c = (t / (E0 * E1)) % E2, d = t / (E0 * E1 * E2);
   const long F0 = g.n[0] + 6, F1 = g.n[1] + 6, F2 = g.n[2] + 6;
-  blk[t] = c4s[(a + 1) + F0 * ((b + 1) + F1 * ((c + 1) + F2 * (d + 1)))];
+  const long o = (a + 1) + F0 * ((b + 1) + F1 * ((c + 1) + F2 * (d + 1)));
+  if (scl) blk[t] = (double)((const long long *)c4s)[o] * scl[1];
+  else blk[t] = c4s[o];
 }
 
 // lambda_rho(j) = sum_i lE_x(i) kx(i - j) + lE_y(i) ky(i - j): the transpose
@@ -528,23 +610,41 @@ void launch_vmoment(const double *in, const double *w, long nxy, long nv, double
 
 template <int BX>
 static void launch_adj_blk(const Geom &g, const double *c4, const double *ex, const double *ey, double dt,
-                           const double *gin, double *lc4, double *lE, bool field_term, hipStream_t s) {
-  const size_t lds = sizeof(double) * ((size_t)(BX + 4) * 6 * 6 * 6 + 2 * (BX + 8) * 10);
+                           const double *gin, double *lc4, double *axy, const double *scl, bool field_term,
+                           hipStream_t s) {
+  const size_t lds = sizeof(unsigned long long) * (size_t)(BX + 4) * 6 * 6 * 6;
   const void *fn = (const void *)k_adj_advect4d_blk<BX>;
   (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const long nb = (long)((g.n[0] + BX - 1) / BX) * ((g.n[1] + 1) / 2) * ((g.n[2] + 1) / 2) * ((g.n[3] + 1) / 2);
   hipLaunchKernelGGL(k_adj_advect4d_blk<BX>, dim3((unsigned)nb), dim3(8 * BX), lds, s, c4, ex, ey, g,
-                     feet_consts(g), dt, gin, lc4, lE, field_term ? 1 : 0);
+                     feet_consts(g), dt, gin, (unsigned long long *)lc4, axy, scl, field_term ? 1 : 0);
+}
+
+bool adj_blocked() {
+  // VLP4D_ADJ_BLOCK=0: the per-cell scatter (floating-point atomics: one per
+  // stencil point and E node) and the half-step scatter
+  if (const char *e = getenv("VLP4D_ADJ_BLOCK")) return atoi(e) != 0;
+  return true;
 }
 
 void launch_adj_advect4d(const Geom &g, const double *c4, const double *ex, const double *ey, double dt,
-                         const double *gin, double *lc4, double *lE, bool field_term, hipStream_t s) {
-  // VLP4D_ADJ_BLOCK=0: the per-cell scatter (one global atomic per stencil point)
-  bool blk = true;
-  if (const char *e = getenv("VLP4D_ADJ_BLOCK")) blk = atoi(e) != 0;
-  if (blk) {
-    if (g.n[0] % 64 == 0) launch_adj_blk<64>(g, c4, ex, ey, dt, gin, lc4, lE, field_term, s);
-    else launch_adj_blk<32>(g, c4, ex, ey, dt, gin, lc4, lE, field_term, s);
+                         const double *gin, double *lc4, double *lE, bool field_term, hipStream_t s,
+                         const AdjWork *wk) {
+  if (wk) {
+    // reproducible form: fixed-point scale from max|g|, blocked scatter,
+    // then the two E gathers
+    const long n = (long)g.n[0] * g.n[1] * g.n[2] * g.n[3];
+    unsigned long long *mx = (unsigned long long *)(wk->scl + 2);
+    hipLaunchKernelGGL(k_fixed_zero, dim3(1), dim3(64), 0, s, mx);
+    hipLaunchKernelGGL(k_absmax_bits, dim3(1024), dim3(256), 0, s, gin, n, mx);
+    hipLaunchKernelGGL(k_fixed_scale, dim3(1), dim3(64), 0, s, (const unsigned long long *)mx, wk->scl);
+    if (g.n[0] % 64 == 0) launch_adj_blk<64>(g, c4, ex, ey, dt, gin, lc4, wk->axy, wk->scl, field_term, s);
+    else launch_adj_blk<32>(g, c4, ex, ey, dt, gin, lc4, wk->axy, wk->scl, field_term, s);
+    if (field_term) {
+      const long np = (long)g.n[0] * g.n[1] * g.n[3], ne = (long)g.n[0] * g.n[1];
+      hipLaunchKernelGGL(k_adjE_x, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, wk->axy, g, dt, wk->part);
+      hipLaunchKernelGGL(k_adjE_y, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, wk->part, g, dt, lE);
+    }
     return;
   }
   const long n = (long)g.n[0] * g.n[1] * g.n[2] * g.n[3];
@@ -574,9 +674,9 @@ void launch_axis_contract(const double *Mt, int Min, int Nout, const double *in,
                      I, accumulate ? 1 : 0);
 }
 
-void launch_c4_block(const Geom &g, const double *c4s, double *blk, hipStream_t s) {
+void launch_c4_block(const Geom &g, const double *c4s, double *blk, hipStream_t s, const double *scl) {
   const long n = (long)(g.n[0] + 4) * (g.n[1] + 4) * (g.n[2] + 4) * (g.n[3] + 4);
-  hipLaunchKernelGGL(k_c4_block, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c4s, blk, g);
+  hipLaunchKernelGGL(k_c4_block, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c4s, blk, g, scl);
 }
 
 void launch_poisson_T(const double *lE, const double *kx, const double *ky, double *lrho, int N0, int N1,
@@ -588,9 +688,7 @@ void launch_poisson_T(const double *lE, const double *kx, const double *ky, doub
 void launch_adj_advect2d(const double *lrho, const Geom &g, double dth, double *lcxy, hipStream_t s) {
   const long n = (long)g.n[0] * g.n[1] * g.n[2] * g.n[3];
   // VLP4D_ADJ_BLOCK=0 also selects the scatter form here (lcxy zeroed by the caller)
-  bool gather = true;
-  if (const char *e = getenv("VLP4D_ADJ_BLOCK")) gather = atoi(e) != 0;
-  if (gather) {
+  if (adj_blocked()) {
     const long ng = (long)(g.n[0] + 4) * (g.n[1] + 4) * g.n[2] * g.n[3];
     hipLaunchKernelGGL(k_adj_advect2d_g, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, lrho, g, dth, lcxy);
     return;

@@ -67,6 +67,7 @@ struct vlp_ctx {
   double *adj_E = nullptr, *adj_rho = nullptr, *adj_k = nullptr, *adj_g = nullptr, *adj_grad = nullptr;
   double *adj_Mt[4] = {};  // M_d^T of the line spline operator, N_d x (N_d + 4)
   double *adj_M[4] = {};   // M_d itself, (N_d + 4) x N_d row-major (the tiled contraction's operand)
+  double *adj_axy = nullptr, *adj_scl = nullptr;  // reproducible scatter: per-cell (ax, ay), fixed-point scale
 };
 
 namespace {
@@ -145,7 +146,8 @@ void free_all(vlp_ctx *c) {
                   c->fb.rho,   c->fb.ex,    c->fb.ey,     c->fb.exy, c->fb.spec1, c->fb.spec2, c->fb.twx,  c->fb.twy,
                   c->fb.diag_part, c->fb.rho_tmp, c->fb.diag, c->d_err, c->adj_c4s, c->adj_blk, c->adj_tmp,
                   c->adj_cxy, c->adj_E, c->adj_rho, c->adj_k, c->adj_g, c->adj_grad, c->adj_Mt[0], c->adj_Mt[1],
-                  c->adj_Mt[2], c->adj_Mt[3], c->adj_M[0], c->adj_M[1], c->adj_M[2], c->adj_M[3], c->lag_tab,
+                  c->adj_Mt[2], c->adj_Mt[3], c->adj_M[0], c->adj_M[1], c->adj_M[2], c->adj_M[3], c->adj_axy,
+                  c->adj_scl, c->lag_tab,
                   c->mf_m[0], c->mf_m[1]};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
@@ -1049,6 +1051,8 @@ static int adjoint_prepare(vlp_ctx *c) {
   ADJ_ALLOC(c->adj_rho, N0 * N1 * sizeof(double));
   ADJ_ALLOC(c->adj_g, c->n_f * sizeof(double));
   ADJ_ALLOC(c->adj_grad, c->n_f * sizeof(double));
+  ADJ_ALLOC(c->adj_axy, 2 * c->n_f * sizeof(double));
+  ADJ_ALLOC(c->adj_scl, 4 * sizeof(double));
   const bool first_mt = c->adj_Mt[0] == nullptr;
   for (int d = 0; d < 4; d++) ADJ_ALLOC(c->adj_Mt[d], (size_t)g.n[d] * (g.n[d] + 4) * sizeof(double));
   for (int d = 0; d < 4; d++) ADJ_ALLOC(c->adj_M[d], (size_t)g.n[d] * (g.n[d] + 4) * sizeof(double));
@@ -1119,12 +1123,16 @@ static int adjoint_core(vlp_ctx *c) {
   int rc = field(c, c->iter + 1, false, false);
   if (rc) return rc;
   run_spline_vxvy<double>(c);
+  // reproducible form (default): fixed-point coefficient sums, E by gathers
+  // (adj_tmp holds the gathers' partial sums until the contractions below)
+  const bool blocked = vlp::adj_blocked();
+  const vlp::AdjWork wk{c->adj_scl, c->adj_axy, c->adj_tmp};
   HIP_TRY(c, hipMemsetAsync(c->adj_c4s, 0, c->n_c4s * sizeof(double), s));
-  HIP_TRY(c, hipMemsetAsync(c->adj_E, 0, 2 * N0 * N1 * sizeof(double), s));
+  if (!blocked) HIP_TRY(c, hipMemsetAsync(c->adj_E, 0, 2 * N0 * N1 * sizeof(double), s));
   vlp::launch_adj_advect4d(g, (const double *)c->c4, c->fb.ex, c->fb.ey, c->cfg.dt, c->adj_g, c->adj_c4s, c->adj_E,
-                           field_term, s);
+                           field_term, s, blocked ? &wk : nullptr);
   // direct term: (M_x^T M_y^T M_vx^T M_vy^T) of the coefficient adjoint
-  vlp::launch_c4_block(g, c->adj_c4s, c->adj_blk, s);
+  vlp::launch_c4_block(g, c->adj_c4s, c->adj_blk, s, blocked ? c->adj_scl : nullptr);
   const bool tiled = adj_tiled();
   vlp::launch_axis_contract(c->adj_Mt[3], (int)E3, (int)N3, c->adj_blk, c->adj_tmp, 1, E0 * E1 * E2, false, s,
                             tiled ? c->adj_M[3] : nullptr);

@@ -137,13 +137,25 @@ size_t lag_tables_bytes(const Geom &g);
 void launch_lag_tables(const Geom &g, void *tab, double dt, hipStream_t s);
 bool launch_lag_xy(const Geom &g, const double *in, double *out, const void *tab, int nchunk, hipStream_t s);
 
+// Work buffers of the reproducible adjoint scatter: scl (4 doubles: the
+// fixed-point scale S, 1/S, and the max|g| bits), axy (2 per cell), part
+// (2 N0 N1 N3).
+struct AdjWork {
+  double *scl, *axy, *part;
+};
+// VLP4D_ADJ_BLOCK unset or nonzero: the reproducible blocked forms
+bool adj_blocked();
+// With wk: lc4 receives 64-bit fixed-point sums (convert with
+// launch_c4_block(..., wk->scl)) and lE is written (not added to); without:
+// floating-point atomics into both.
 void launch_adj_advect4d(const Geom &g, const double *c4, const double *ex, const double *ey, double dt,
-                         const double *gin, double *lc4, double *lE, bool field_term, hipStream_t s);
+                         const double *gin, double *lc4, double *lE, bool field_term, hipStream_t s,
+                         const AdjWork *wk = nullptr);
 // out[o][n][i] (+)= sum_m Mt[n][m] in[o][m][i]; with M (= Mt^T, Min x Nout
 // row-major) given, by LDS tiles of 64 columns
 void launch_axis_contract(const double *Mt, int Min, int Nout, const double *in, double *out, long O, long I,
                           bool accumulate, hipStream_t s, const double *M = nullptr);
-void launch_c4_block(const Geom &g, const double *c4s, double *blk, hipStream_t s);
+void launch_c4_block(const Geom &g, const double *c4s, double *blk, hipStream_t s, const double *scl = nullptr);
 void launch_poisson_T(const double *lE, const double *kx, const double *ky, double *lrho, int N0, int N1,
                       hipStream_t s);
 void launch_adj_advect2d(const double *lrho, const Geom &g, double dth, double *lcxy, hipStream_t s);

@@ -13,8 +13,12 @@ The forward step it differentiates is itself held to the oracle at 1e-12
   term through the Poisson solve and the half step is exercised) agree with
   <grad, d>;
 * a torch.autograd.Function that binds the step and its adjoint passes
-  torch.autograd.gradcheck (fast mode; the adjoint sums with atomic adds,
-  whose order varies from run to run at round-off, hence nondet_tol).
+  torch.autograd.gradcheck (fast mode);
+* the adjoint is reproducible: the coefficient adjoint is summed as 64-bit
+  fixed-point integers and the E adjoint by gathers, so repeated calls (and
+  the checkpointed sweep) return the same bits; the per-cell
+  floating-point-atomic form (VLP4D_ADJ_BLOCK=0) agrees with it to
+  round-off.
 
 The velocity foot is clamped to the box, max(min(v - dt E, v_max), v_min)
 (VMO/Advection.hpp:214-217), which has no derivative where v - dt E meets a
@@ -159,6 +163,26 @@ def test_adjoint_leaves_the_run_unchanged(vlp):
         da, db = a.diags(), b.diags()
         for key in ("nrj", "nrjx", "nrjy", "mass", "l2norm"):
             np.testing.assert_array_equal(da[key], db[key])
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_adjoint_is_reproducible(vlp, monkeypatch, k):
+    n, idcase, vmax, lx = NONLINEAR[k]
+    rng = np.random.default_rng(11 + k)
+    g = rng.standard_normal(n[::-1])
+    with _sim(vlp, n, idcase, vmax, lx, nbiter=8) as sim:
+        sim.step(2)
+        a = sim.adjoint_step(g)
+        b = sim.adjoint_step(g)
+        np.testing.assert_array_equal(a, b)
+        s3a = sim.adjoint_steps(3, g)
+        s3b = sim.adjoint_steps(3, g)
+        np.testing.assert_array_equal(s3a, s3b)
+        # the floating-point-atomic form gives the same gradient to round-off
+        monkeypatch.setenv("VLP4D_ADJ_BLOCK", "0")
+        c = sim.adjoint_step(g)
+        monkeypatch.delenv("VLP4D_ADJ_BLOCK")
+        np.testing.assert_allclose(c, a, rtol=0, atol=1e-12 * np.abs(a).max())
 
 
 def test_adjoint_rejects_bad_requests(vlp):
