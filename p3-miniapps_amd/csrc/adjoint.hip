@@ -463,19 +463,31 @@ __global__ __launch_bounds__(256) void k_c4_block(const double *__restrict__ c4s
 __global__ __launch_bounds__(256) void k_poisson_T(const double *__restrict__ lE, const double *__restrict__ kx,
                                                    const double *__restrict__ ky, double *__restrict__ lrho, int N0,
                                                    int N1) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= N0 * N1) return;
-  const int jx = t % N0, jy = t / N0;
+  // 16 outputs per workgroup, each summed over 16 slices of the source rows
+  // by 16 lanes, then the slices added in a fixed order
+  __shared__ double part[16][16];
+  const int o = threadIdx.x & 15, p = threadIdx.x >> 4;
+  const long t = (long)blockIdx.x * 16 + o;
   double s = 0.;
-  for (int iy = 0; iy < N1; iy++) {
-    const int dy = (iy - jy + N1) % N1;
-    for (int ix = 0; ix < N0; ix++) {
-      const int dx = (ix - jx + N0) % N0;
-      const long i = (long)iy * N0 + ix, k = (long)dy * N0 + dx;
-      s += lE[2 * i] * kx[k] + lE[2 * i + 1] * ky[k];
+  if (t < (long)N0 * N1) {
+    const int jx = (int)(t % N0), jy = (int)(t / N0);
+    const int y0 = (int)((long)p * N1 / 16), y1 = (int)((long)(p + 1) * N1 / 16);
+    for (int iy = y0; iy < y1; iy++) {
+      const int dy = iy >= jy ? iy - jy : iy - jy + N1;
+      const double *e = lE + 2 * (long)iy * N0;
+      const double *kxr = kx + (long)dy * N0, *kyr = ky + (long)dy * N0;
+      // ix = 0 .. N0-1 with dx = (ix - jx) mod N0, without a division per term
+      for (int ix = 0; ix < jx; ix++) s += e[2 * ix] * kxr[ix - jx + N0] + e[2 * ix + 1] * kyr[ix - jx + N0];
+      for (int ix = jx; ix < N0; ix++) s += e[2 * ix] * kxr[ix - jx] + e[2 * ix + 1] * kyr[ix - jx];
     }
   }
-  lrho[t] = s;
+  part[p][o] = s;
+  __syncthreads();
+  if (p == 0 && t < (long)N0 * N1) {
+    double r = 0.;
+    for (int q = 0; q < 16; q++) r += part[q][o];
+    lrho[t] = r;
+  }
 }
 
 // Through the half step: f_half(x, y, v) = I2(C_xy) at (x - dt/2 vx,
@@ -681,8 +693,7 @@ void launch_c4_block(const Geom &g, const double *c4s, double *blk, hipStream_t 
 
 void launch_poisson_T(const double *lE, const double *kx, const double *ky, double *lrho, int N0, int N1,
                       hipStream_t s) {
-  hipLaunchKernelGGL(k_poisson_T, dim3((unsigned)((N0 * N1 + 255) / 256)), dim3(256), 0, s, lE, kx, ky, lrho, N0,
-                     N1);
+  hipLaunchKernelGGL(k_poisson_T, dim3((unsigned)((N0 * N1 + 15) / 16)), dim3(256), 0, s, lE, kx, ky, lrho, N0, N1);
 }
 
 void launch_adj_advect2d(const double *lrho, const Geom &g, double dth, double *lcxy, hipStream_t s) {
