@@ -215,6 +215,42 @@ def ensemble_bench(pkg, steps, seed=20261202):
     }
 
 
+def adjoint_bench(pkg, steps, n=96, every=10, seed=20261203):
+    """Gradient of the integrated field energy J = dt sum_k |E(f^k)|^2 dx dy
+    over `steps` steps (vlp_adjoint_field_energy, FP64) from a Landau state
+    at n^4 (box 4 pi x 4 pi x [-6, 6]^2, dt 0.02, idcase 10) plus a seeded
+    density perturbation M(v) (P delta)(x, y), delta on a 64 x 64 grid, P the
+    trigonometric interpolation to n x n: the sweep of the adjoint workload
+    test, timed against the same number of plain steps.  Only the velocity
+    moment of the gradient (weight M) crosses to the host."""
+    import numpy as np
+    nb = 2 * steps + 2
+    cfg = pkg.VlpConfig.make([n] * 4, [0.0, 0.0, -6.0, -6.0], [4 * math.pi, 4 * math.pi, 6.0, 6.0], 10, 0.02, nb)
+    rng = np.random.default_rng(seed)
+    delta = rng.normal(0.0, 1e-3, (64, 64))
+    P = pkg.trig_interp_matrix(64, n)
+    v = -6.0 + np.arange(n) * (12.0 / n)
+    M = np.exp(-0.5 * (v[:, None] ** 2 + v[None, :] ** 2)) / (2 * math.pi)
+    with pkg.Vlp4d(cfg) as sim:
+        sim.set_f(sim.get_f() + M[:, :, None, None] * (P @ delta @ P.T)[None, None, :, :])
+        sim.field_energy_gradient(1, vweight=M, checkpoint_every=1)  # warm-up: operators, buffers
+        t0 = time.perf_counter()
+        J, _ = sim.field_energy_gradient(steps, vweight=M, checkpoint_every=every)
+        t_adj = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sim.step(steps)
+        t_fwd = time.perf_counter() - t0
+    return {
+        "metric": f"adjoint field-energy sweep ({steps} steps at {n}^4, FP64, checkpoint every {every})",
+        "value": t_adj, "unit": "s", "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": 1e3 * t_adj / steps,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": f"synthetic: Landau state + density perturbation on a 64 x 64 grid, seed {seed}",
+        "config": {"workload": f"vlp_adjoint_field_energy over {steps} steps, {n}^4, velocity moment of the gradient",
+                   "grid": [n] * 4, "checkpoint_every": every},
+        "J": J, "forward_s": t_fwd, "sweep_over_forward": t_adj / t_fwd,
+    }
+
+
 def lagrange_cpu_baseline(n, steps=1):
     """The Lagrange oracle (C/OpenMP port of miniapps/vlp4d/openmp) on the
     host cores: `steps` onetimestep(s) of the same deck, timed in-process."""
@@ -348,6 +384,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--ensemble", action="store_true",
                     help="batched-ensemble workload (48 uneven Landau solves); --steps is the solve length")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="the adjoint field-energy sweep at 96^4 (FP64); --steps is the sweep length")
     ap.add_argument("--lagrange", action="store_true",
                     help="the Lagrange split-step solver of miniapps/vlp4d on the same deck")
     a = ap.parse_args()
@@ -361,6 +399,9 @@ def main():
     pkg = load_pkg()
     if a.ensemble:
         print(json.dumps(ensemble_bench(pkg, a.steps)))
+        return
+    if a.adjoint:
+        print(json.dumps(adjoint_bench(pkg, a.steps)))
         return
     if a.lagrange:
         print(json.dumps(lagrange_bench(pkg, a.n, a.steps, a.warmup, not a.no_cpu_baseline)))
