@@ -95,10 +95,11 @@ __global__ __launch_bounds__(XY_NT) void k_xy_adv2d(const T *__restrict__ f, T *
       }
     }
     __syncthreads();
-    // 2. y recursions on columns 0..E0-1
+    // 2. y recursions on columns 0..E0-1 (a wave stops when its first
+    // column is past the plane: its lines are whole within the wave)
     {
-      const int py = tid & (Py - 1), yl = tid / Py;
-      for (int c0 = 0; c0 < E0; c0 += XY_NT / Py) {
+      const int py = tid & (Py - 1), yl = tid / Py, yl0 = (wave * 64) / Py;
+      for (int c0 = 0; c0 + yl0 < E0; c0 += XY_NT / Py) {
         const int xc = c0 + yl;
         const bool act = xc < E0;
         double w[C];
@@ -123,11 +124,43 @@ __global__ __launch_bounds__(XY_NT) void k_xy_adv2d(const T *__restrict__ f, T *
       }
     }
     __syncthreads();
-    // 3. store the CXY plane (positions -2..N+1 in x and y)
+    // 3. store the CXY plane (positions -2..N+1 in x and y).  The plane is
+    // contiguous in global memory, so it leaves as 16-byte chunks, lane after
+    // lane (row and column of a chunk tracked incrementally); per element
+    // when the rows do not split into whole chunks.
     {
       T *dst = cxy + ((long)ivx + (long)N2 * ivy) * E0 * E1;
-      for (int yr = wave; yr < E1; yr += XY_NT / 64)
-        for (int xc = lane; xc < E0; xc += 64) dst[(long)yr * E0 + xc] = pl[yr * R + xc];
+      constexpr int VPC = 16 / (int)sizeof(T);
+      if (E0 % VPC == 0 && ((size_t)dst & 15) == 0) {
+        const int cpr = E0 / VPC, nch = cpr * E1;
+        const int drow = XY_NT / cpr, dcol = XY_NT - drow * cpr;
+        int row = tid / cpr, col = tid - row * cpr;
+        for (int k = tid; k < nch; k += XY_NT) {
+          const T *s = pl + row * R + VPC * col;
+          if constexpr (VPC == 4) {
+            float4 v;
+            v.x = s[0];
+            v.y = s[1];
+            v.z = s[2];
+            v.w = s[3];
+            *(float4 *)(dst + (long)VPC * k) = v;
+          } else {
+            double2 v;
+            v.x = s[0];
+            v.y = s[1];
+            *(double2 *)(dst + (long)VPC * k) = v;
+          }
+          row += drow;
+          col += dcol;
+          if (col >= cpr) {
+            col -= cpr;
+            row++;
+          }
+        }
+      } else {
+        for (int yr = wave; yr < E1; yr += XY_NT / 64)
+          for (int xc = lane; xc < E0; xc += 64) dst[(long)yr * E0 + xc] = pl[yr * R + xc];
+      }
     }
     // 4. 2-D advection at (x - dt/2 vx, y - dt/2 vy), interp_2D's spline branch
     if (aact) {
