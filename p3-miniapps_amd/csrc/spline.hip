@@ -270,6 +270,11 @@ __global__ __launch_bounds__(256) void k_spline_line_scan(const T *__restrict__ 
 // is contiguous across i1 as well (is1 == X0) the tiles run over the combined
 // index i0 + X0 i1 and each tile column finds its output position once.
 typedef __attribute__((address_space(3))) void lds_void_t;
+// 16-byte chunks: aligned in LDS, element-aligned in global memory
+typedef float f4a __attribute__((ext_vector_type(4)));
+typedef double d2a __attribute__((ext_vector_type(2)));
+typedef float f4e __attribute__((ext_vector_type(4), aligned(4)));
+typedef double d2e __attribute__((ext_vector_type(2), aligned(8)));
 
 template <typename T, int C, int XT>
 __global__ __launch_bounds__(XT == 256 ? 1024 : 512) void k_spline_tile(const T *__restrict__ in, T *__restrict__ out, LineMap m, int P,
@@ -325,12 +330,34 @@ __global__ __launch_bounds__(XT == 256 ? 1024 : 512) void k_spline_tile(const T 
     }
   }
   __syncthreads();
-  // N+4 coefficient rows out
+  // N+4 coefficient rows out.  When the contiguous extent X0 splits into
+  // whole 16-byte chunks, no chunk straddles two runs of X0, so a chunk stays
+  // contiguous in the output: a lane keeps one chunk column of the tile and
+  // writes it down the rows, one 16-byte LDS read and one multi-dword store
+  // per row (output rows are only element-aligned, which global stores
+  // allow).  Else element by element.
   T *dst = out + i1 * m.os1 + o * m.oos;
-  for (int x = lane; x < nx; x += 64) {
-    const long J = i0 + x;
-    const long ox = comb ? (J % m.X0) + (J / m.X0) * m.os1 : J;
-    for (int k = wave; k < N + 4; k += nw) dst[(long)k * m.ols + ox] = tile[k * XT + x];
+  if (m.X0 % VPL == 0) {
+    const int c = tid % CPR, rpp = (int)blockDim.x / CPR;
+    if (c * VPL < nx) {
+      const long J = i0 + (long)c * VPL;
+      const long ox = comb ? (J % m.X0) + (J / m.X0) * m.os1 : J;
+      for (int k = tid / CPR; k < N + 4; k += rpp) {
+        const T *s = tile + k * XT + c * VPL;
+        T *d = dst + (long)k * m.ols + ox;
+        if constexpr (VPL == 4) {
+          *(f4e *)d = *(const f4a *)s;
+        } else {
+          *(d2e *)d = *(const d2a *)s;
+        }
+      }
+    }
+  } else {
+    for (int x = lane; x < nx; x += 64) {
+      const long J = i0 + x;
+      const long ox = comb ? (J % m.X0) + (J / m.X0) * m.os1 : J;
+      for (int k = wave; k < N + 4; k += nw) dst[(long)k * m.ols + ox] = tile[k * XT + x];
+    }
   }
 }
 
