@@ -64,7 +64,7 @@ typedef __attribute__((address_space(3))) void lds_void;
 // weights cy) on (y, vy), which is uniform in the workgroup at a step.  A lane
 // therefore keeps G[k1] = sum_k0 cx[k0] E(ip0+k0, ip1+k1) for the three rows
 // and, as ip1 advances by one per step, computes only the new row.
-template <typename T, typename A, int TVX, int NB> struct YStream {
+template <typename T, typename A, int TVX, int NB, int RXC> struct YStream {
   static constexpr int VPL = 16 / (int)sizeof(T);  // slab values per lane load
   // double arithmetic reads the foot constants from LDS (see open())
   static constexpr bool KLDS = sizeof(A) == sizeof(double);
@@ -285,15 +285,22 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
   // The plane's contribution t[c] to each of the four open cells from the 16
   // rows at LDS element sp (one stencil key): rows (kvx, kvy) in order, the
   // next row's LDS reads issued before the current row's FMAs (one row of
-  // look-ahead hides the LDS latency).
+  // look-ahead hides the LDS latency).  With the slab row pitch known at
+  // compile time (RXC = N0 + 4), rows kvx = 0, 1 and 2, 3 of a vy row are
+  // read from one base address each, the second row through the immediate
+  // offsets of the paired LDS reads (an empty asm keeps the compiler from
+  // rebuilding a base per row): 8 address additions per pass instead of 16.
   __device__ __forceinline__ void pass_rows(const T *sp, A t[4]) const {
-    const int VS = (TVX + 4) * RX;  // vy row stride
+    const int VS = (TVX + 4) * (RXC ? RXC : RX);  // vy row stride
+    typedef const __attribute__((address_space(3))) T lds_t;
+    lds_t *pb = (lds_t *)sp;
+    if constexpr (RXC != 0) asm volatile("" : "+v"(pb));
     A u[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) t[c] = u[c] = 0;
     T r[2][4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) r[0][k] = sp[k];
+    for (int k = 0; k < 4; k++) r[0][k] = RXC ? pb[k] : sp[k];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
       const int kvx = i & 3, kvy = i >> 2;
@@ -301,9 +308,20 @@ template <typename T, typename A, int TVX, int NB> struct YStream {
       // flight), then row i+1's reads go out ahead of row i's FMAs
       __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
       if (i + 1 < 16) {
-        const T *rn = sp + ((i + 1) >> 2) * VS + ((i + 1) & 3) * RX;
+        const int n = i + 1;
+        if constexpr (RXC != 0) {
+          if ((n & 1) == 0) {  // first row of a pair: a new base
+            pb = (lds_t *)(sp + (n >> 2) * VS + (n & 2) * RXC);
+            asm volatile("" : "+v"(pb));
+          }
+          const int o = (n & 1) * RXC;
 #pragma unroll
-        for (int k = 0; k < 4; k++) r[(i + 1) & 1][k] = rn[k];
+          for (int k = 0; k < 4; k++) r[n & 1][k] = pb[o + k];
+        } else {
+          const T *rn = sp + (n >> 2) * VS + (n & 3) * RX;
+#pragma unroll
+          for (int k = 0; k < 4; k++) r[n & 1][k] = rn[k];
+        }
       }
       __builtin_amdgcn_sched_barrier(0);
       const A r0 = (A)r[i & 1][0], r1 = (A)r[i & 1][1], r2 = (A)r[i & 1][2], r3 = (A)r[i & 1][3];
@@ -441,13 +459,13 @@ __device__ __forceinline__ void ys_block_tile(int b, int grid, int ntile, int N3
 
 }  // namespace
 
-template <typename T, typename A, int TVX, int NB>
+template <typename T, typename A, int TVX, int NB, int RXC>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(A) == 4 ? 3 : 1)))
 void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geom g,
                                                      FeetConsts fc, double dt, T *__restrict__ fout,
                                                      int *__restrict__ err, int force, int remap) {
   extern __shared__ __align__(16) unsigned char ys_smem[];
-  YStream<T, A, TVX, NB> s;
+  YStream<T, A, TVX, NB, RXC> s;
   s.force = force;
   s.c4 = c4;
   s.exy = exy;
@@ -632,25 +650,34 @@ YsPlan ys_plan(const Geom &g, size_t tsz, int want_tvx = 0, int want_nb = 0) {
   return p;
 }
 
-template <typename T, typename A, int TVX, int NB>
+template <typename T, typename A, int TVX, int NB, int RXC>
 void launch_ys_t(const Geom &g, const YsPlan &p, const T *c4, const double *exy, double dt, T *fout, int *err,
                  hipStream_t s) {
-  const void *fn = (const void *)k_advect4d_ys<T, A, TVX, NB>;
+  const void *fn = (const void *)k_advect4d_ys<T, A, TVX, NB, RXC>;
   if (p.lds > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
   const unsigned grid = (unsigned)((g.n[2] / TVX) * g.n[3]);
   int force = 0;  // test switch: VLP4D_YS_FORCE=1 rows per cell, 2 direct gather, 3 both
   if (const char *e = getenv("VLP4D_YS_FORCE")) force = atoi(e) & 3;
   int remap = 1;  // VLP4D_YS_REMAP=0: block b -> (b mod ntile, b / ntile)
   if (const char *e = getenv("VLP4D_YS_REMAP")) remap = atoi(e) != 0;
-  hipLaunchKernelGGL((k_advect4d_ys<T, A, TVX, NB>), dim3(grid), dim3(p.block), p.lds, s, c4, exy, g,
+  hipLaunchKernelGGL((k_advect4d_ys<T, A, TVX, NB, RXC>), dim3(grid), dim3(p.block), p.lds, s, c4, exy, g,
                      feet_consts(g), dt, fout, err, force, remap);
 }
 
 template <typename T, typename A, int TVX>
 void launch_ys_nb(const Geom &g, const YsPlan &p, const T *c4, const double *exy, double dt, T *fout, int *err,
                   hipStream_t s) {
-  if (p.nb == 2) launch_ys_t<T, A, TVX, 2>(g, p, c4, exy, dt, fout, err, s);
-  else launch_ys_t<T, A, TVX, 3>(g, p, c4, exy, dt, fout, err, s);
+  // N0 = 128 with the vx tile of 2 (the tile chosen at 128^4) takes the
+  // kernel with the slab row pitch 132 built in
+  if constexpr (TVX == 2) {
+    if (g.n[0] == 128) {
+      if (p.nb == 2) launch_ys_t<T, A, TVX, 2, 132>(g, p, c4, exy, dt, fout, err, s);
+      else launch_ys_t<T, A, TVX, 3, 132>(g, p, c4, exy, dt, fout, err, s);
+      return;
+    }
+  }
+  if (p.nb == 2) launch_ys_t<T, A, TVX, 2, 0>(g, p, c4, exy, dt, fout, err, s);
+  else launch_ys_t<T, A, TVX, 3, 0>(g, p, c4, exy, dt, fout, err, s);
 }
 
 }  // namespace

@@ -287,7 +287,13 @@ __global__ __launch_bounds__(XT == 256 ? 1024 : 512) void k_spline_tile(const T 
   const bool comb = m.is1 == m.X0;
   const long XC = comb ? m.X0 * m.X1 : m.X0;  // contiguous input extent
   const long ntile = (XC + XT - 1) / XT;
-  const long tl = blockIdx.x % ntile, ob = blockIdx.x / ntile;
+  // consecutive tiles, whose rows share memory pages, on one XCD: blocks go
+  // to the 8 XCDs round-robin, so block b takes logical tile (b mod 8) G/8 +
+  // b/8 over the first 8 floor(G/8) blocks (the rest in order); each XCD's
+  // address-translation cache then serves a run of neighbouring tiles
+  const unsigned G8 = gridDim.x & ~7u, bk = blockIdx.x;
+  const long lb = bk < G8 ? (long)(bk & 7u) * (G8 >> 3) + (bk >> 3) : (long)bk;
+  const long tl = lb % ntile, ob = lb / ntile;
   const long i1 = comb ? 0 : ob % m.X1, o = comb ? ob : ob / m.X1;
   const long i0 = tl * XT;
   const int nx = (int)((XC - i0) < XT ? (XC - i0) : XT);
