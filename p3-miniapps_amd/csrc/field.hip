@@ -107,20 +107,26 @@ __global__ __launch_bounds__(256) void k_poisson_y(Geom g, FieldBufs b) {
   }
   __syncthreads();
   for (int iy = threadIdx.x; iy < N1; iy += blockDim.x) {
-    for (int fld = 0; fld < 3; fld++) {
-      const double *h = hat + 2 * N1 * fld;
-      double sr = 0., si = 0.;
-      int m = 0;
-      for (int q = 0; q < N1; q++) {
-        const double wr = b.twy[2 * m], wi = -b.twy[2 * m + 1];  // exp(+2 pi i m/N)
-        sr += h[2 * q] * wr - h[2 * q + 1] * wi;
-        si += h[2 * q] * wi + h[2 * q + 1] * wr;
-        m += iy;
-        if (m >= N1) m -= N1;
+    // the three fields side by side: one twiddle load per term and three
+    // independent sums (each in the same order as alone)
+    double sr[3] = {0., 0., 0.}, si[3] = {0., 0., 0.};
+    int m = 0;
+    for (int q = 0; q < N1; q++) {
+      const double wr = b.twy[2 * m], wi = -b.twy[2 * m + 1];  // exp(+2 pi i m/N)
+#pragma unroll
+      for (int fld = 0; fld < 3; fld++) {
+        const double *h = hat + 2 * N1 * fld;
+        sr[fld] += h[2 * q] * wr - h[2 * q + 1] * wi;
+        si[fld] += h[2 * q] * wi + h[2 * q + 1] * wr;
       }
+      m += iy;
+      if (m >= N1) m -= N1;
+    }
+#pragma unroll
+    for (int fld = 0; fld < 3; fld++) {
       double *dst = b.spec2 + 2 * ((long)nxh * N1 * fld + k + (long)nxh * iy);
-      dst[0] = sr;
-      dst[1] = si;
+      dst[0] = sr[fld];
+      dst[1] = si[fld];
     }
   }
 }
@@ -141,20 +147,25 @@ __global__ __launch_bounds__(256) void k_c2rx_diag(Geom g, FieldBufs b) {
   __syncthreads();
   double pm = 0., pe = 0., pex = 0., pey = 0.;
   for (int ix = threadIdx.x; ix < N0; ix += blockDim.x) {
+    // the three fields side by side (one twiddle load per term, each sum in
+    // the same order as alone)
     double out[3];
-    for (int fld = 0; fld < 3; fld++) {
-      const double *X = rows + 2 * nxh * fld;
-      double s = X[0];
-      int m = ix;
-      for (int k = 1; k < (N0 + 1) / 2; k++) {
-        const double wr = b.twx[2 * m], wi = -b.twx[2 * m + 1];
-        s += 2.0 * (X[2 * k] * wr - X[2 * k + 1] * wi);
-        m += ix;
-        if (m >= N0) m -= N0;
+#pragma unroll
+    for (int fld = 0; fld < 3; fld++) out[fld] = rows[2 * nxh * fld];
+    int m = ix;
+    for (int k = 1; k < (N0 + 1) / 2; k++) {
+      const double wr = b.twx[2 * m], wi = -b.twx[2 * m + 1];
+#pragma unroll
+      for (int fld = 0; fld < 3; fld++) {
+        const double *X = rows + 2 * nxh * fld;
+        out[fld] += 2.0 * (X[2 * k] * wr - X[2 * k + 1] * wi);
       }
-      if ((N0 & 1) == 0) s += X[2 * (N0 / 2)] * ((ix & 1) ? -1.0 : 1.0);
-      out[fld] = s;
+      m += ix;
+      if (m >= N0) m -= N0;
     }
+    if ((N0 & 1) == 0)
+#pragma unroll
+      for (int fld = 0; fld < 3; fld++) out[fld] += rows[2 * nxh * fld + 2 * (N0 / 2)] * ((ix & 1) ? -1.0 : 1.0);
     const long id = ix + (long)N0 * iy;
     b.rho[id] = out[0];
     b.ex[id] = out[1];
@@ -228,6 +239,9 @@ __global__ __launch_bounds__(256) void k_diag_final(Geom g, FieldBufs b, const d
   double l2 = 0.;
   if (with_l2) {
     double v = 0.;
+    // unrolled so that the independent loads are in flight together (the
+    // additions keep their order)
+#pragma unroll 8
     for (long i = threadIdx.x; i < n_l2; i += blockDim.x) v += l2_part[i];
     l2 = block_sum(v, red);
   }
