@@ -56,6 +56,7 @@ struct vlp_ctx {
   double tsec[VLP_T_COUNT] = {};
   int tcalls[VLP_T_COUNT] = {};
   hipEvent_t ev[8] = {};
+  hipEvent_t evb[8] = {};  // second set: several steps in one call alternate between ev and evb
   int ys_tvx = 0, ys_nb = 0;  // 4-D advection tile chosen by vlp_autotune (0: preference order)
   int sp_mode[2] = {vlp::SPLINE_AUTO, vlp::SPLINE_AUTO};  // vx / vy spline kernels chosen by vlp_autotune
   vlp_ensemble *owner = nullptr;  // the ensemble whose captured graph holds this context's buffers
@@ -153,6 +154,8 @@ void free_all(vlp_ctx *c) {
     if (p) (void)hipFree(p);
   if (c->h_err) (void)hipHostFree(c->h_err);
   for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : c->evb)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -300,10 +303,10 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
 // between the phases, diagnostics at index iter + 1.  Under graph capture
 // (iter_dev non-null): no events, the diagnostics index comes from the device
 // counter *iter_dev (advanced by the field solve).
-int enqueue_step(vlp_ctx *c, int *iter_dev) {
+int enqueue_step(vlp_ctx *c, int *iter_dev, hipEvent_t *evs = nullptr) {
   const bool f64 = c->precision == VLP_FP64;
   const bool ev_on = iter_dev == nullptr;
-  hipEvent_t *ev = c->ev;
+  hipEvent_t *ev = evs ? evs : c->ev;
   if (ev_on) HIP_TRY(c, hipEventRecord(ev[0], c->stream));
   if (f64) run_spline_xy<double>(c); else run_spline_xy<float>(c);
   if (ev_on) HIP_TRY(c, hipEventRecord(ev[1], c->stream));
@@ -383,14 +386,8 @@ int lag_step_once(vlp_ctx *c) {
 }
 
 // One onetimestep + swap, events between phases.
-int step_once(vlp_ctx *c) {
-  if (c->scheme == VLP_SCHEME_LAGRANGE) return lag_step_once(c);
-  hipEvent_t *ev = c->ev;
-  int rc = enqueue_step(c, nullptr);
-  if (rc) return rc;
-  c->iter++;
-  rc = sync_and_check(c);
-  if (rc) return rc;
+// Phase timers of one spline step from its events ev[0..6].
+void add_step_timers(vlp_ctx *c, hipEvent_t *ev) {
   c->tsec[VLP_T_SPLINE_XY] += 1e-3 * ev_ms(ev[0], ev[1]);
   c->tsec[VLP_T_ADVEC2D] += 1e-3 * ev_ms(ev[1], ev[2]);
   c->tsec[VLP_T_FIELD] += 1e-3 * (ev_ms(ev[2], ev[3]) + ev_ms(ev[5], ev[6]));
@@ -407,6 +404,56 @@ int step_once(vlp_ctx *c) {
   c->tcalls[VLP_T_PACK]++;
   c->tcalls[VLP_T_COMM]++;
   c->tcalls[VLP_T_UNPACK]++;
+}
+
+int step_once(vlp_ctx *c) {
+  if (c->scheme == VLP_SCHEME_LAGRANGE) return lag_step_once(c);
+  int rc = enqueue_step(c, nullptr);
+  if (rc) return rc;
+  c->iter++;
+  rc = sync_and_check(c);
+  if (rc) return rc;
+  add_step_timers(c, c->ev);
+  return VLP_OK;
+}
+
+// Several spline steps: step s + 1 is enqueued before the host reads step s's
+// error flag and events, so the GPU does not idle between steps while the
+// host synchronises.  Step s uses event set s mod 2 (ev, evb) and flag slot
+// s mod 2; its copy of the (sticky) device flag and a last event are queued
+// behind its kernels.  A CFL error found at step s marks the context broken
+// with iter at that step (the step already queued after it ran on its
+// result, as the reference would not, but the context is unusable then).
+int steps_pipelined(vlp_ctx *c, int nsteps) {
+  hipEvent_t *sets[2] = {c->ev, c->evb};
+  auto t_last = std::chrono::steady_clock::now();
+  int pend = -1;
+  for (int s = 0; s <= nsteps; s++) {
+    const int k = s & 1;
+    if (s < nsteps) {
+      const int rc = enqueue_step(c, nullptr, sets[k]);
+      if (rc) return rc;
+      c->iter++;
+      HIP_TRY(c, hipMemcpyAsync(c->h_err + k, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(c, hipEventRecord(sets[k][7], c->stream));
+    }
+    if (pend >= 0) {
+      HIP_TRY(c, hipEventSynchronize(sets[pend][7]));
+      if (c->h_err[pend]) {
+        c->broken = true;
+        if (s < nsteps) c->iter--;
+        return set_err(c, VLP_E_CFL, "Time step is too large, exiting");
+      }
+      add_step_timers(c, sets[pend]);
+      const auto now = std::chrono::steady_clock::now();
+      const double dt = std::chrono::duration<double>(now - t_last).count();
+      t_last = now;
+      c->tsec[VLP_T_MAINLOOP] += dt;
+      c->tcalls[VLP_T_MAINLOOP]++;
+      c->tsec[VLP_T_TOTAL] += dt;
+    }
+    pend = s < nsteps ? k : -1;
+  }
   return VLP_OK;
 }
 
@@ -578,6 +625,8 @@ static int create_ctx(const vlp_config *cfg_in, int precision, int device, int s
     return fail(set_err(nullptr, VLP_E_HIP, "hipStreamCreate failed"));
   for (auto &e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) return fail(set_err(nullptr, VLP_E_HIP, "hipEventCreate failed"));
+  for (auto &e : c->evb)
+    if (hipEventCreate(&e) != hipSuccess) return fail(set_err(nullptr, VLP_E_HIP, "hipEventCreate failed"));
   ALLOC(c->f, c->n_f * c->esz);
   ALLOC(c->fnext, c->n_f * c->esz);
   if (lag) ALLOC(c->lag_tab, vlp::lag_tables_bytes(c->g));
@@ -601,7 +650,7 @@ static int create_ctx(const vlp_config *cfg_in, int precision, int device, int s
   ALLOC(c->fb.diag, 5 * (size_t)(cfg.nbiter + 1) * sizeof(double));
   ALLOC(c->d_err, sizeof(int));
 #undef ALLOC
-  if (hipHostMalloc((void **)&c->h_err, sizeof(int), hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc((void **)&c->h_err, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
     return fail(set_err(nullptr, VLP_E_HIP, "hipHostMalloc failed"));
   c->fb.nbiter = cfg.nbiter;
   const std::vector<double> twx = twiddles((int)N0), twy = twiddles((int)N1);
@@ -667,6 +716,7 @@ int vlp_step(vlp_ctx *c, int nsteps) {
   if (rc) return rc;
   if (nsteps < 0 || c->iter + nsteps > c->cfg.nbiter)
     return set_err(c, VLP_E_ARG, "step count exceeds nbiter of the configuration");
+  if (c->scheme == VLP_SCHEME_SPLINE && nsteps > 1) return steps_pipelined(c, nsteps);
   for (int s = 0; s < nsteps; s++) {
     const auto t0 = std::chrono::steady_clock::now();
     rc = step_once(c);
