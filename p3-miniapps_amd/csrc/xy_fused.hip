@@ -9,7 +9,8 @@
 //   1. x recursions: f rows straight from global memory (each lane a chunk of
 //      C values, the chunked scan of spline.hip), coefficients into LDS rows;
 //   2. y recursions on the LDS columns, in place;
-//   3. the CXY plane (positions -2..N+1 in x and y) leaves LDS as whole rows;
+//   3. the CXY plane (positions -2..N+1 in x and y) leaves LDS as 16-byte
+//      non-temporal stores;
 //   4. the 2-D advection of every (x, y) of the plane reads the stencil from
 //      LDS; f at the half step is never stored, it is summed over the
 //      workgroup's vx planes into density partials.
@@ -126,8 +127,10 @@ __global__ __launch_bounds__(XY_NT) void k_xy_adv2d(const T *__restrict__ f, T *
     __syncthreads();
     // 3. store the CXY plane (positions -2..N+1 in x and y).  The plane is
     // contiguous in global memory, so it leaves as 16-byte chunks, lane after
-    // lane (row and column of a chunk tracked incrementally); per element
-    // when the rows do not split into whole chunks.
+    // lane (row and column of a chunk tracked incrementally), as
+    // non-temporal stores (the plane is read again only by the vx spline
+    // pass, after the whole kernel); per element when the rows do not split
+    // into whole chunks.
     {
       T *dst = cxy + ((long)ivx + (long)N2 * ivy) * E0 * E1;
       constexpr int VPC = 16 / (int)sizeof(T);
@@ -137,19 +140,11 @@ __global__ __launch_bounds__(XY_NT) void k_xy_adv2d(const T *__restrict__ f, T *
         int row = tid / cpr, col = tid - row * cpr;
         for (int k = tid; k < nch; k += XY_NT) {
           const T *s = pl + row * R + VPC * col;
-          if constexpr (VPC == 4) {
-            float4 v;
-            v.x = s[0];
-            v.y = s[1];
-            v.z = s[2];
-            v.w = s[3];
-            *(float4 *)(dst + (long)VPC * k) = v;
-          } else {
-            double2 v;
-            v.x = s[0];
-            v.y = s[1];
-            *(double2 *)(dst + (long)VPC * k) = v;
-          }
+          typedef T chunk_t __attribute__((ext_vector_type(VPC)));
+          chunk_t v;
+#pragma unroll
+          for (int e = 0; e < VPC; e++) v[e] = s[e];
+          __builtin_nontemporal_store(v, (chunk_t *)(dst + (long)VPC * k));
           row += drow;
           col += dcol;
           if (col >= cpr) {
