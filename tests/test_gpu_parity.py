@@ -124,6 +124,41 @@ def test_cfl_violation_reports_error(vlp):
         assert ei.value.code == vlp.E_CFL
 
 
+@pytest.mark.parametrize("precision", ["FP64", "FP32_ACC"])
+def test_multistep_call_equals_single_steps(vlp, precision):
+    # vlp_step(n > 1) queues step s + 1 before it reads step s's error flag
+    # and phase events; f, the diagnostics and the timer counts must be
+    # those of n calls of one step.
+    args = sld_box((24, 20, 16, 18), vmax=6.0, dt=0.05, nbiter=7)
+    prec = getattr(vlp, precision)
+    with vlp.Vlp4d(vlp.VlpConfig.make(*args), precision=prec) as a, \
+            vlp.Vlp4d(vlp.VlpConfig.make(*args), precision=prec) as b:
+        a.step(7)
+        for _ in range(7):
+            b.step(1)
+        assert a.iter == b.iter == 7
+        assert np.array_equal(a.get_f(), b.get_f())
+        da, db = a.diags(), b.diags()
+        for k in da:
+            assert np.array_equal(da[k], db[k]), k
+        ta, tb = a.timers(), b.timers()
+        assert all(ta[k][1] == tb[k][1] for k in ta)
+        assert ta["MainLoop"][1] == 7 and ta["advec4D"][0] > 0
+
+
+def test_cfl_violation_in_a_multistep_call(vlp):
+    # the failing step is the first of three: the error names it, iter stays
+    # at that step and the context refuses further steps
+    args = sld_box(16, vmax=6.0, dt=3 * (4 * math.pi / 16) / 6.0, nbiter=3)
+    with vlp.Vlp4d(vlp.VlpConfig.make(*args)) as sim:
+        with pytest.raises(vlp.Vlp4dError) as ei:
+            sim.step(3)
+        assert ei.value.code == vlp.E_CFL
+        assert sim.iter == 1
+        with pytest.raises(vlp.Vlp4dError):
+            sim.step(1)
+
+
 # The vx-blocked 4-D advection (taken when the y-streaming kernel does not
 # fit the geometry, forced here with VLP4D_YSTREAM=0) has tuned variants picked
 # at launch from the environment (cells per lane block, E rows staged in LDS
