@@ -134,7 +134,10 @@ int vlp_destroy(vlp_ctx *ctx);
 
 /* `nsteps` iterations of onetimestep (VMO/Timestep.hpp:34-97) followed by
  * fn.swap(fnp1) (VMO/Vlp4d.cpp:137-147).  Returns VLP_E_CFL if a foot left the
- * domain (state is then invalid). */
+ * domain (state is then invalid; vlp_iter names the failing step).  With
+ * nsteps > 1 (spline solver) step s + 1 is queued before step s's error flag
+ * is read, so the GPU does not wait for the host between steps; results are
+ * those of nsteps calls of one step. */
 int vlp_step(vlp_ctx *ctx, int nsteps);
 
 /* Current iteration count (the reference's `iter`). */
