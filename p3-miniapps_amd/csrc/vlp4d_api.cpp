@@ -425,6 +425,16 @@ int step_once(vlp_ctx *c) {
 // with iter at that step (the step already queued after it ran on its
 // result, as the reference would not, but the context is unusable then).
 int steps_pipelined(vlp_ctx *c, int nsteps) {
+  // a HIP failure here leaves queued steps of unknown outcome: the context
+  // is marked unusable
+#define HIP_TRY_STEP(expr)                                                                  \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      c->broken = true;                                                                     \
+      return set_err(c, VLP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    }                                                                                       \
+  } while (0)
   hipEvent_t *sets[2] = {c->ev, c->evb};
   auto t_last = std::chrono::steady_clock::now();
   int pend = -1;
@@ -432,13 +442,16 @@ int steps_pipelined(vlp_ctx *c, int nsteps) {
     const int k = s & 1;
     if (s < nsteps) {
       const int rc = enqueue_step(c, nullptr, sets[k]);
-      if (rc) return rc;
+      if (rc) {
+        c->broken = true;
+        return rc;
+      }
       c->iter++;
-      HIP_TRY(c, hipMemcpyAsync(c->h_err + k, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      HIP_TRY(c, hipEventRecord(sets[k][7], c->stream));
+      HIP_TRY_STEP(hipMemcpyAsync(c->h_err + k, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY_STEP(hipEventRecord(sets[k][7], c->stream));
     }
     if (pend >= 0) {
-      HIP_TRY(c, hipEventSynchronize(sets[pend][7]));
+      HIP_TRY_STEP(hipEventSynchronize(sets[pend][7]));
       if (c->h_err[pend]) {
         c->broken = true;
         if (s < nsteps) c->iter--;
@@ -454,6 +467,7 @@ int steps_pipelined(vlp_ctx *c, int nsteps) {
     }
     pend = s < nsteps ? k : -1;
   }
+#undef HIP_TRY_STEP
   return VLP_OK;
 }
 
