@@ -56,3 +56,30 @@ def test_fp32_storage_error_budget(vlp, orc, mode, idcase, n, steps, vmax, lx):
     phi = st.field(0)
     noise = 2.2e-16 * np.sum(np.abs(phi)) * lx / n[0] * lx / n[1]
     assert np.max(np.abs(d["mass"])) < 16 * noise
+
+
+@pytest.mark.parametrize("mode", ["FP32", "FP32_ACC"])
+def test_fp32_storage_on_a_128_point_x_axis(vlp, orc, mode):
+    # N0 = 128 runs the y-streaming 4-D advection instantiated with the slab
+    # row pitch 132 built in (one LDS base per pair of stencil rows) and the
+    # fused x/y kernel on 128-point rows; same budgets as above
+    n, steps, vmax, lx = (128, 16, 12, 10), 4, 6.0, 4 * math.pi
+    dt = 0.5 * (lx / n[0]) / vmax
+    args = sld_box(n, vmax=vmax, dt=dt, nbiter=steps, idcase=10, lx=lx)
+    st = orc.State(orc.make_config(*args))
+    st.step(steps)
+    ref_f, ref_d = st.f_interior(), st.diags()
+    with vlp.Vlp4d(vlp.VlpConfig.make(*args), precision=getattr(vlp, mode)) as sim:
+        assert sim.kernel_variants() & (1 << 7)  # y-streaming 4-D advection
+        assert sim.kernel_variants() & (1 << 12)  # fused x/y kernel
+        sim.step(steps)
+        f = sim.get_f()
+        d = sim.diags()
+    ef = relmax(f, ref_f)
+    e_nrj = float(np.max(np.abs(np.exp(d["nrj"] - ref_d["nrj"]) - 1)))
+    e_l2 = float(np.max(np.abs(d["l2norm"] / ref_d["l2norm"] - 1)))
+    print(f"{mode} vs fp64 oracle {n} x{steps}: f {ef:.2e}  |E| {e_nrj:.2e}  l2 {e_l2:.2e}")
+    bf, be, bl = BUDGET[mode]
+    assert ef < bf
+    assert e_nrj < be
+    assert e_l2 < bl
