@@ -10,7 +10,9 @@ namespace vlp {
 // (VMO/Efield.cpp:54-106) with the FFT conventions of lib/OpenMP_FFT.hpp:
 // unnormalised r2c along x, c2c along y, normalisation in k-space, inverse
 // c2c along y, c2r along x (imaginary parts of DC/Nyquist ignored).  The
-// transforms are direct DFTs with exactly reduced twiddle tables.
+// transforms are direct DFTs with exactly reduced twiddle tables; their loops
+// are unrolled so that several terms' twiddle loads are in flight (each sum
+// keeps its order).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_rho_rfftx(const double *__restrict__ rho_part, long rho_outer, Geom g,
                                                    FieldBufs b, int sub1) {
@@ -20,6 +22,7 @@ __global__ __launch_bounds__(256) void k_rho_rfftx(const double *__restrict__ rh
   const int nxh = N0 / 2 + 1;
   for (int ix = threadIdx.x; ix < N0; ix += blockDim.x) {
     double s = 0.;
+#pragma unroll 8
     for (long o = 0; o < rho_outer; o++) s += rho_part[ix + (long)N0 * (iy + (long)N1 * o)];
     double r = s * g.dx[2] * g.dx[3];
     if (sub1) r -= 1.;
@@ -29,6 +32,7 @@ __global__ __launch_bounds__(256) void k_rho_rfftx(const double *__restrict__ rh
   for (int k = threadIdx.x; k < nxh; k += blockDim.x) {
     double sr = 0., si = 0.;
     int m = 0;
+#pragma unroll 8
     for (int j = 0; j < N0; j++) {
       sr += row[j] * b.twx[2 * m];
       si += row[j] * b.twx[2 * m + 1];
@@ -70,6 +74,7 @@ __global__ __launch_bounds__(256) void k_poisson_y(Geom g, FieldBufs b) {
   for (int q = threadIdx.x; q < N1; q += blockDim.x) {
     double cr = 0., ci = 0.;
     int m = 0;
+#pragma unroll 8
     for (int j = 0; j < N1; j++) {
       const double wr = b.twy[2 * m], wi = b.twy[2 * m + 1];
       cr += col[2 * j] * wr - col[2 * j + 1] * wi;
@@ -111,6 +116,7 @@ __global__ __launch_bounds__(256) void k_poisson_y(Geom g, FieldBufs b) {
     // independent sums (each in the same order as alone)
     double sr[3] = {0., 0., 0.}, si[3] = {0., 0., 0.};
     int m = 0;
+#pragma unroll 8
     for (int q = 0; q < N1; q++) {
       const double wr = b.twy[2 * m], wi = -b.twy[2 * m + 1];  // exp(+2 pi i m/N)
 #pragma unroll
@@ -153,6 +159,7 @@ __global__ __launch_bounds__(256) void k_c2rx_diag(Geom g, FieldBufs b) {
 #pragma unroll
     for (int fld = 0; fld < 3; fld++) out[fld] = rows[2 * nxh * fld];
     int m = ix;
+#pragma unroll 8
     for (int k = 1; k < (N0 + 1) / 2; k++) {
       const double wr = b.twx[2 * m], wi = -b.twx[2 * m + 1];
 #pragma unroll
@@ -198,6 +205,7 @@ __global__ __launch_bounds__(256) void k_analytic_field(const double *__restrict
   double pm = 0., pe = 0., pex = 0., pey = 0.;
   for (int ix = threadIdx.x; ix < N0; ix += blockDim.x) {
     double s = 0.;
+#pragma unroll 8
     for (long o = 0; o < rho_outer; o++) s += rho_part[ix + (long)N0 * (iy + (long)N1 * o)];
     const double r = s * g.dx[2] * g.dx[3];
     const double e = (g.idcase == 2) ? -(g.minPhy[0] + ix * g.dx[0]) : -(g.minPhy[1] + iy * g.dx[1]);
