@@ -178,9 +178,10 @@ __global__ __launch_bounds__(256) void k_lag_v(const double *__restrict__ in, do
 // operations execute in order); y pass from LDS to HBM.  The weights come
 // from the context's tables (k_lag_tables): the x ones (x, vx) into registers
 // once per plane -- a lane's x positions are fixed -- and the y ones (y, vy)
-// as wave-uniform reads when a wave covers part of one row; the LDS holds
-// only the plane.  Each cell's arithmetic is that of k_lag_x followed by
-// k_lag_y.
+// as wave-uniform reads when a wave's points lie in one row; the LDS holds
+// only the plane.  The plane's loads, its LDS copy and the y pass work on
+// pairs of points (16 bytes per lane).  Each cell's arithmetic is that of
+// k_lag_x followed by k_lag_y.
 constexpr int LAG_XY_THREADS = 1024;
 constexpr int LAG_XY_MAXM = 16;  // plane points per lane: planes up to 16384 points
 
@@ -222,18 +223,22 @@ __global__ __launch_bounds__(LAG_XY_THREADS) void k_lag_xy(const double *__restr
   const int l = threadIdx.x, wave = l >> 6, lane = l & 63;
   constexpr int NW = LAG_XY_THREADS / 64;
   const LagTab *ytab = tab + (size_t)N2 * N0 + (size_t)ivy * N1;
-  // with N0 a multiple of 64 every wave of the y pass covers part of one row:
-  // its weights are wave-uniform
-  const bool row_waves = (N0 & 63) == 0;
-  const int dix = LAG_XY_THREADS % N0, diy = LAG_XY_THREADS / N0;
+  // points go in pairs (x, x+1), 16 bytes per lane for the plane's loads,
+  // its LDS copy, the y pass's reads and its stores (N0 even, checked by the
+  // launcher); with N0 a multiple of 128 a wave's 128 points of the y pass
+  // lie in one row, whose weights are then wave-uniform
+  constexpr int MH = LAG_XY_MAXM / 2;
+  const bool row_waves2 = (N0 & 127) == 0;
+  const int dix = (2 * LAG_XY_THREADS) % N0, diy = (2 * LAG_XY_THREADS) / N0;
+  typedef double d2v __attribute__((ext_vector_type(2)));
   // the plane in flight: loaded during the previous plane's y pass
-  double r[LAG_XY_MAXM];
+  d2v r[MH];
   {
     const double *src = in + ((size_t)ivy * N2 + v0) * (size_t)P;
 #pragma unroll
-    for (int m = 0; m < LAG_XY_MAXM; m++) {
-      const int p = l + LAG_XY_THREADS * m;
-      r[m] = p < P ? src[p] : 0.;
+    for (int m = 0; m < MH; m++) {
+      const int p = 2 * (l + LAG_XY_THREADS * m);
+      r[m] = p < P ? *(const d2v *)(src + p) : d2v{0., 0.};
     }
   }
   for (int ivx = v0; ivx < v1; ivx++) {
@@ -247,9 +252,9 @@ __global__ __launch_bounds__(LAG_XY_THREADS) void k_lag_xy(const double *__restr
     }
     __syncthreads();  // the previous plane's y pass has read pl
 #pragma unroll
-    for (int m = 0; m < LAG_XY_MAXM; m++) {
-      const int p = l + LAG_XY_THREADS * m;
-      if (p < P) pl[p] = r[m];
+    for (int m = 0; m < MH; m++) {
+      const int p = 2 * (l + LAG_XY_THREADS * m);
+      if (p < P) *(d2v *)(pl + p) = r[m];
     }
     __syncthreads();
     for (int row = wave; row < N1; row += NW) {
@@ -280,25 +285,27 @@ __global__ __launch_bounds__(LAG_XY_THREADS) void k_lag_xy(const double *__restr
     if (ivx + 1 < v1) {  // next plane's loads, in flight during the y pass
       const double *nx = in + off + (size_t)P;
 #pragma unroll
-      for (int m = 0; m < LAG_XY_MAXM; m++) {
-        const int p = l + LAG_XY_THREADS * m;
-        if (p < P) r[m] = nx[p];
+      for (int m = 0; m < MH; m++) {
+        const int p = 2 * (l + LAG_XY_THREADS * m);
+        if (p < P) r[m] = *(const d2v *)(nx + p);
       }
     }
     double *dst = out + off;
-    int ix = l % N0, iy = l / N0;
+    int ix = (2 * l) % N0, iy = (2 * l) / N0;
 #pragma unroll 4
-    for (int p = l; p < P; p += LAG_XY_THREADS) {
-      const int ty = row_waves ? __builtin_amdgcn_readfirstlane(iy) : iy;
+    for (int p = 2 * l; p < P; p += 2 * LAG_XY_THREADS) {
+      const int ty = row_waves2 ? __builtin_amdgcn_readfirstlane(iy) : iy;
       const LagTab &t = ytab[ty];
-      double s = 0.;
+      d2v s = {0., 0.};
 #pragma unroll
       for (int k = 0; k < 6; k++) {
         int q = t.b + k;
         q = q >= N1 ? q - N1 : q;
-        s += t.w[k] * pl[q * N0 + ix];
+        const d2v v = *(const d2v *)(pl + q * N0 + ix);
+        s.x += t.w[k] * v.x;
+        s.y += t.w[k] * v.y;
       }
-      dst[p] = s;
+      *(d2v *)(dst + p) = s;
       ix += dix;
       iy += diy;
       if (ix >= N0) {
@@ -317,7 +324,8 @@ int lag_xy_chunks(const Geom &g) {
   if (const char *e = getenv("VLP4D_LAG_FUSED"))
     if (atoi(e) == 0) return 0;
   // rows of up to 128 points (two 64-lane chunks of x weights in registers)
-  if (g.n[0] > 128 || (long)g.n[0] * g.n[1] > (long)LAG_XY_THREADS * LAG_XY_MAXM || lag_xy_lds(g) > 160 * 1024)
+  if (g.n[0] > 128 || (g.n[0] & 1) || (long)g.n[0] * g.n[1] > (long)LAG_XY_THREADS * LAG_XY_MAXM ||
+      lag_xy_lds(g) > 160 * 1024)
     return 0;
   const int want = (1024 + g.n[3] - 1) / g.n[3];
   const int G = (g.n[2] + want - 1) / want;  // vx planes per workgroup
