@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_lag_v(const double *__restrict__ in, do
   double w0 = src[(size_t)j[0] * sD], w1 = src[(size_t)j[1] * sD], w2 = src[(size_t)j[2] * sD];
   double w3 = src[(size_t)j[3] * sD], w4 = src[(size_t)j[4] * sD], w5 = src[(size_t)j[5] * sD];
   int nxt = j[5];  // index of the window's last point
-#pragma unroll 4
+#pragma unroll 8
   for (int iv = 0; iv < ND; iv++) {
     double s = c[0] * w0;
     s += c[1] * w1;
