@@ -209,8 +209,10 @@ const char *vlp_last_error(const vlp_ctx *ctx);
  * buffers) and the kernel of the vx and of the vy spline pass (256-, 128- or
  * 64-position LDS tiles, or the line kernels): every choice the geometry
  * admits is timed on the context's own buffers and the fastest is used by its
- * later steps; csv_path (optional) receives the timings, one row per choice
- * with the chosen ones marked.  Follows the reference's tile tuner, which
+ * later steps when it beats the default choice (the preference order used
+ * without tuning) by more than 2 %, else the default stays; csv_path
+ * (optional) receives the timings, one row per choice with the default and
+ * the chosen ones marked.  Follows the reference's tile tuner, which
  * scans the candidate tiles of every registered kernel, keeps the minimum and
  * logs a CSV (miniapps/vlp4d_mpi/kokkos/Tile_size_tuning.hpp:51-96,127-148;
  * per-kernel registration miniapps/vlp4d_mpi/kokkos/Tuning.hpp:26-65).  The

@@ -704,6 +704,11 @@ bool ystream_eligible(const Geom &g, size_t tsz, int tvx, int nb) {
   return p.tvx == tvx && p.nb == nb;
 }
 
+bool ystream_preferred(const Geom &g, size_t tsz, int tvx, int nb) {
+  const YsPlan p = ys_plan(g, tsz);
+  return p.tvx == tvx && p.nb == nb;
+}
+
 int ystream_tvx(const Geom &g, bool fp32_storage, bool arith32) {
   if (const char *e = getenv("VLP4D_YSTREAM"))
     if (atoi(e) == 0) return 0;

@@ -972,8 +972,9 @@ int vlp_kernel_variants(const vlp_ctx *c) {
 // kernels: every (vx tile, plane buffers) this geometry admits, and every
 // distinct spline kernel choice, is launched on the context's own buffers
 // (one warm-up and three timed launches, HIP events on its stream) and the
-// fastest is kept for the context's later steps.  Same scan-and-keep-the-
-// minimum scheme as the reference's tile tuner
+// fastest is kept for the context's later steps if it beats the preference
+// order's choice by more than 2 % (else that choice stays).  Same scan-and-
+// keep-the-minimum scheme as the reference's tile tuner
 // (miniapps/vlp4d_mpi/kokkos/Tile_size_tuning.hpp:51-96), with its CSV log
 // (:127-148).  The launches write fnext, which
 // the next step overwrites, and report errors to a scratch flag.
@@ -985,11 +986,16 @@ int vlp_autotune(vlp_ctx *c, const char *csv_path) {
   const bool ys_env = getenv("VLP4D_YS_TVX") || getenv("VLP4D_YS_NBUF") || getenv("VLP4D_YSTREAM");
   const bool sp_env = getenv("VLP4D_SPLINE_TILE") || getenv("VLP4D_SPLINE_WIDE");
   const size_t tsz = c->precision == VLP_FP64 ? sizeof(double) : sizeof(float);
-  struct Cand { int tvx, nb; float ms; };
+  // A choice replaces the preference order's (the incumbent) only when it is
+  // faster by more than this fraction: back-to-back launches of one kernel
+  // differ by about 1 %, and near-ties do not carry over to the full step.
+  constexpr float kMargin = 0.02f;
+  struct Cand { int tvx, nb; bool pref; float ms; };
   std::vector<Cand> cands;
   for (int tvx : {2, 4, 8})
     for (int nb : {2, 3})
-      if (!ys_env && vlp::ystream_eligible(c->g, tsz, tvx, nb)) cands.push_back({tvx, nb, 0.f});
+      if (!ys_env && vlp::ystream_eligible(c->g, tsz, tvx, nb))
+        cands.push_back({tvx, nb, vlp::ystream_preferred(c->g, tsz, tvx, nb), 0.f});
   // spline passes: each distinct kernel choice (128- or 64-position tiles,
   // line kernels) of the vx and the vy pass, on cxy -> tv and tv -> c4
   // (both overwritten by the next step)
@@ -1032,10 +1038,13 @@ int vlp_autotune(vlp_ctx *c, const char *csv_path) {
     if (b) (void)hipEventDestroy(b);
     if (!sok) return set_err(c, VLP_E_HIP, "autotune launch failed (spline)");
     for (int pass = 0; pass < 2; pass++) {
-      const SCand *bs = nullptr;
+      const SCand *inc = nullptr, *bs = nullptr;  // SPLINE_AUTO (listed first) is the incumbent
       for (const auto &k : spl)
-        if (k.pass == pass && (!bs || k.ms < bs->ms)) bs = &k;
-      if (bs) c->sp_mode[pass] = bs->mode;
+        if (k.pass == pass) {
+          if (!inc) inc = &k;
+          else if (!bs || k.ms < bs->ms) bs = &k;
+        }
+      if (inc) c->sp_mode[pass] = bs && bs->ms < (1.f - kMargin) * inc->ms ? bs->mode : inc->mode;
     }
   }
   const Cand *best = nullptr;
@@ -1068,22 +1077,26 @@ int vlp_autotune(vlp_ctx *c, const char *csv_path) {
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(d_scratch);
     if (!ok) return set_err(c, VLP_E_HIP, "autotune launch failed");
-    best = &cands[0];
+    const Cand *inc = &cands[0];
     for (const auto &k : cands)
-      if (k.ms < best->ms) best = &k;
+      if (k.pref) inc = &k;
+    best = inc;
+    for (const auto &k : cands)
+      if (k.ms < (1.f - kMargin) * inc->ms && k.ms < best->ms) best = &k;
     c->ys_tvx = best->tvx;
     c->ys_nb = best->nb;
   }
   if (csv_path && *csv_path) {
     FILE *f = fopen(csv_path, "w");
     if (!f) return set_err(c, VLP_E_FILE, std::string("cannot write ") + csv_path);
-    fprintf(f, "kernel,nx,ny,nvx,nvy,precision,vx_tile,plane_buffers,ms_per_launch,chosen\n");
+    fprintf(f, "kernel,nx,ny,nvx,nvy,precision,vx_tile,plane_buffers,ms_per_launch,preferred,chosen\n");
     for (const auto &k : cands)
-      fprintf(f, "advect4d,%d,%d,%d,%d,%d,%d,%d,%.6f,%d\n", c->g.n[0], c->g.n[1], c->g.n[2], c->g.n[3], c->precision,
-              k.tvx, k.nb, k.ms, &k == best ? 1 : 0);
+      fprintf(f, "advect4d,%d,%d,%d,%d,%d,%d,%d,%.6f,%d,%d\n", c->g.n[0], c->g.n[1], c->g.n[2], c->g.n[3],
+              c->precision, k.tvx, k.nb, k.ms, k.pref ? 1 : 0, &k == best ? 1 : 0);
     for (const auto &k : spl)
-      fprintf(f, "spline_%s,%d,%d,%d,%d,%d,%d,%d,%.6f,%d\n", k.pass ? "vy" : "vx", c->g.n[0], c->g.n[1], c->g.n[2],
-              c->g.n[3], c->precision, k.width, 0, k.ms, k.mode == c->sp_mode[k.pass] ? 1 : 0);
+      fprintf(f, "spline_%s,%d,%d,%d,%d,%d,%d,%d,%.6f,%d,%d\n", k.pass ? "vy" : "vx", c->g.n[0], c->g.n[1],
+              c->g.n[2], c->g.n[3], c->precision, k.width, 0, k.ms, k.mode == vlp::SPLINE_AUTO ? 1 : 0,
+              k.mode == c->sp_mode[k.pass] ? 1 : 0);
     fclose(f);
   }
   return VLP_OK;

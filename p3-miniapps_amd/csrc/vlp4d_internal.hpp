@@ -101,6 +101,9 @@ bool launch_advect4d_ystream(const Geom &g, const T *c4, const double *exy, doub
 // Whether the streaming kernel takes (tvx, nb) for this geometry and storage
 // size (the autotuner's candidate filter).
 bool ystream_eligible(const Geom &g, size_t tsz, int tvx, int nb);
+// Whether (tvx, nb) is the preference order's choice for this geometry (the
+// autotuner's incumbent).
+bool ystream_preferred(const Geom &g, size_t tsz, int tvx, int nb);
 // vx tile of the y-streaming kernel for a geometry (0: not used).
 int ystream_tvx(const Geom &g, bool fp32_storage, bool arith32);
 template <typename T>
