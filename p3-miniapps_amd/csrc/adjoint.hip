@@ -38,7 +38,7 @@ __device__ __forceinline__ void bspline_deta(double w, double d[4]) {
 
 // Per cell: the feet of the forward step, then
 //  * g * w0 w1 w2 w3 added to lc4 at the 4^4 stencil (lc4 has the C4
-//    storage layout: positions -3..N+2, stride N_d + 6);
+//    storage layout, pitches c4_pitch);
 //  * the foot term g * dI4/dfoot * dfoot/dEbar, scattered to lE (Ex, Ey
 //    interleaved, N1 x N0) with computeFeet's Lagrange weights.
 __global__ __launch_bounds__(256) void k_adj_advect4d(const double *__restrict__ c4, const double *__restrict__ ex,
@@ -68,7 +68,8 @@ __global__ __launch_bounds__(256) void k_adj_advect4d(const double *__restrict__
     bspline_eta<double>(fj - ip[j], w[j]);
     bspline_deta(fj - ip[j], dw[j]);
   }
-  const long S1 = N0 + 6, S2 = S1 * (N1 + 6), S3 = S2 * (N2 + 6);
+  const C4Pitch cp = c4_pitch(N0, N1, N2);
+  const long S1 = cp.s1, S2 = cp.s2, S3 = cp.s3;
   const long base = (ip[0] + 2) + S1 * (ip[1] + 2) + S2 * (ip[2] + 2) + S3 * (long)(ip[3] + 2);
   double dI[4] = {0., 0., 0., 0.};
   for (int k3 = 0; k3 < 4; k3++)
@@ -159,7 +160,8 @@ __global__ __launch_bounds__(8 * BX) void k_adj_advect4d_blk(const double *__res
   __syncthreads();
   const int lx = tid % BX, r = tid / BX;
   const int ix = x0 + lx, iy = y0 + (r & 1), ivx = vx0 + ((r >> 1) & 1), ivy = vy0 + (r >> 2);
-  const long S1 = N0 + 6, S2 = S1 * (N1 + 6), S3 = S2 * (N2 + 6);
+  const C4Pitch cp = c4_pitch(N0, N1, N2);
+  const long S1 = cp.s1, S2 = cp.s2, S3 = cp.s3;
   if (ix < N0 && iy < N1 && ivx < N2 && ivy < N3) {
     const long t = ix + (long)N0 * (iy + (long)N1 * (ivx + (long)N2 * ivy));
     const double gv = gin[t];
@@ -451,8 +453,8 @@ __global__ __launch_bounds__(256) void k_c4_block(const double *__restrict__ c4s
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= E0 * E1 * E2 * E3) return;
   const long a = t % E0, b = (t / E0) % E1, c = (t / (E0 * E1)) % E2, d = t / (E0 * E1 * E2);
-  const long F0 = g.n[0] + 6, F1 = g.n[1] + 6, F2 = g.n[2] + 6;
-  const long o = (a + 1) + F0 * ((b + 1) + F1 * ((c + 1) + F2 * (d + 1)));
+  const C4Pitch cp = c4_pitch(g.n[0], g.n[1], g.n[2]);
+  const long o = (a + 1) + cp.s1 * (b + 1) + cp.s2 * (c + 1) + cp.s3 * (d + 1);
   if (scl) blk[t] = (double)((const long long *)c4s)[o] * scl[1];
   else blk[t] = c4s[o];
 }

@@ -144,7 +144,8 @@ __global__ __launch_bounds__(256) void k_advect4d_blk(const T *__restrict__ c4, 
   double l2 = 0.;
   int bad = 0;
   if (active) {
-    const long S1 = N0 + 6, S2 = S1 * (N1 + 6), S3 = S2 * (N2 + 6);
+    const C4Pitch cp = c4_pitch(N0, N1, N2);
+    const long S1 = cp.s1, S2 = cp.s2, S3 = cp.s3;
     const double x = g.minPhy[0] + ix * g.dx[0];
     T *orow = fout + ix + (long)N0 * (iy + (long)N1 * (long)N2 * ivy);
     const long ostride = (long)N0 * N1;

@@ -122,10 +122,11 @@ template <typename T> __device__ __forceinline__ void load_row5(const T *p, T r[
   r[4] = p[4];
 }
 
-// Direct 4^4 gather of interp_4D on the C4 layout (positions -3..N+2, stride
-// F_d = N_d + 6).  A valid foot has ip in [-1, N], so positions ip-1..ip+2
-// stay inside -3..N+2; any reading of -3 or N+2 carries weight 0 (or ~1e-16
-// at the lower edge), where the stored value is 0.
+// Direct 4^4 gather of interp_4D on the C4 layout (positions -3..N+2, pitches
+// c4_pitch).  A valid foot has ip in [-1, N], so positions ip-1..ip+2 stay
+// inside -3..N+2; any reading of -3 or N+2 carries weight 0 (or ~1e-16 at the
+// lower edge).  Along y, vx and vy the stored value there is 0; along x that
+// slot overlaps the neighbouring row, whose finite value the weight cancels.
 template <typename T, typename A>
 __device__ __forceinline__ A gather4d(const T *__restrict__ c4, long S1, long S2, long S3, const int ip[4],
                                       const A eta[4][4]) {

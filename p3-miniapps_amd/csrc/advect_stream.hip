@@ -269,8 +269,8 @@ template <typename T, typename A, int TVX, int NB, int RXC> struct YStream {
       // kept in scalar registers across the loop
       int n0 = N0, n1 = N1, n2 = N2;
       asm volatile("" : "+s"(n0), "+s"(n1), "+s"(n2));
-      const long s1 = n0 + 6, s2 = s1 * (n1 + 6), s3 = s2 * (n2 + 6);
-      c.acc = gather4d<T, A>(c4, s1, s2, s3, ip, e);
+      const C4Pitch cp = c4_pitch(n0, n1, n2);
+      c.acc = gather4d<T, A>(c4, cp.s1, cp.s2, cp.s3, ip, e);
 #pragma unroll
       for (int k = 0; k < 4; k++) c.w[0][k] = 0;  // no share of a common pass
     }
@@ -531,23 +531,25 @@ void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geo
   s.vy = g.minPhy[3] + s.ivy * g.dx[3];
   s.oyw = s.vy > 0. ? -1 : 0;
   s.fbase = fout + s.ix + (long)s.N0 * s.N1 * (s.ivx + (long)s.N2 * s.ivy);
-  s.F0 = s.N0 + 6;
-  s.S1 = s.N0 + 6;
-  s.S2 = s.S1 * (s.N1 + 6);
-  s.S3 = s.S2 * (s.N2 + 6);
+  {
+    const C4Pitch cp = c4_pitch(s.N0, s.N1, s.N2);
+    s.F0 = (int)cp.s1;
+    s.S1 = cp.s1;
+    s.S2 = cp.s2;
+    s.S3 = cp.s3;
+  }
   s.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   s.lane = tid & 63;
   s.nwave = (int)(blockDim.x >> 6);
   s.nslot = s.wave < ninstr ? (ninstr - 1 - s.wave) / s.nwave + 1 : 0;
   {
-    const long F01 = (long)s.F0 * (s.N1 + 6);
 #pragma unroll
     for (int k = 0; k < s.MS; k++) {
       // chunks past the slab repeat its last chunk into the buffer's padding
       const int c = min(64 * (s.wave + k * s.nwave) + s.lane, nchunk - 1);
       const int row = c / per_row, col = c - row * per_row;
       const int kvx = row % (TVX + 4), kvy = row / (TVX + 4);
-      s.voff[k] = (unsigned)((F01 * (kvx + (long)(s.N2 + 6) * kvy) + (long)s.VPL * col) * (long)sizeof(T));
+      s.voff[k] = (unsigned)((s.S2 * kvx + s.S3 * kvy + (long)s.VPL * col) * (long)sizeof(T));
     }
   }
   {  // x part of computeFeet's E interpolation (VMO/Advection.hpp:174-196)
@@ -563,7 +565,7 @@ void k_advect4d_ys(const T *__restrict__ c4, const double *__restrict__ exy, Geo
   }
   s.ip1p = -(1 << 30);
   // position (x -2, y -3, vx tile-2, vy ivy-2) sits at storage (1, 0, tile+1, ivy+1)
-  s.cbase = 1 + s.F0 * (s.N1 + 6) * ((tile * TVX + 1) + (s.N2 + 6) * (s.ivy + 1));
+  s.cbase = (int)(1 + s.S2 * (tile * TVX + 1) + s.S3 * (s.ivy + 1));
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     s.cell[c].kbit = 0;
@@ -616,7 +618,7 @@ long ys_epr(long N0) { return (N0 + 63) / 64 * 64; }
 YsPlan ys_plan(const Geom &g, size_t tsz, int want_tvx = 0, int want_nb = 0) {
   YsPlan p;
   const long N0 = g.n[0], N1 = g.n[1], N2 = g.n[2], N3 = g.n[3];
-  const long c4n = (N0 + 6) * (N1 + 6) * (N2 + 6) * (N3 + 6) + 64;
+  const long c4n = c4_elems(N0, N1, N2, N3);
   if (N0 % 2 || c4n >= (1L << 31)) return p;
   int force = want_tvx, force_nb = want_nb;
   if (const char *e = getenv("VLP4D_YS_TVX")) force = atoi(e);

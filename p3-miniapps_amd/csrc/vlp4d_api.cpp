@@ -40,8 +40,9 @@ struct vlp_ctx {
   hipStream_t stream = nullptr;
   size_t esz = 8;
   void *f = nullptr, *fnext = nullptr, *cxy = nullptr, *tv = nullptr, *c4 = nullptr;
+  void *c4_alloc = nullptr;  // allocation holding c4 (c4_shift elements in)
   size_t n_f = 0, n_cxy = 0, n_tv = 0, n_c4 = 0;
-  size_t n_c4s = 0;  // stored C4: positions -3..N+2 per dim plus a zeroed tail
+  size_t n_c4s = 0;  // stored C4 (layout of c4_pitch) plus a zeroed tail
   double *rho_part = nullptr, *l2_part = nullptr;
   long n_l2 = 0;
   int xy_groups = 0;        // G of the fused xy kernel (0: separate kernels)
@@ -143,7 +144,7 @@ std::vector<double> twiddles(int n) {
 }
 
 void free_all(vlp_ctx *c) {
-  void *bufs[] = {c->f,        c->fnext,    c->cxy,       c->tv,     c->c4,      c->rho_part, c->l2_part,
+  void *bufs[] = {c->f,        c->fnext,    c->cxy,       c->tv,     c->c4_alloc, c->rho_part, c->l2_part,
                   c->fb.rho,   c->fb.ex,    c->fb.ey,     c->fb.exy, c->fb.spec1, c->fb.spec2, c->fb.twx,  c->fb.twy,
                   c->fb.diag_part, c->fb.rho_tmp, c->fb.diag, c->d_err, c->adj_c4s, c->adj_blk, c->adj_tmp,
                   c->adj_cxy, c->adj_E, c->adj_rho, c->adj_k, c->adj_g, c->adj_grad, c->adj_Mt[0], c->adj_Mt[1],
@@ -208,22 +209,31 @@ vlp::LineMap vx_map(const Geom &g) {
   return m;
 }
 
-// vy-pass: tv (E0, E1, E2, N3) -> c4 (F0, F1, F2, F3) at positions -2..N+1
+// C4 element of position (-2, -2, -2, -2), the first coefficient the vy pass
+// writes, and the shift of C4 in its allocation that puts it on a 64-byte
+// boundary (the pitches past x are multiples of 16 elements).
+long c4_origin(const Geom &g) {
+  const vlp::C4Pitch cp = vlp::c4_pitch(g.n[0], g.n[1], g.n[2]);
+  return 1 + cp.s1 + cp.s2 + cp.s3;
+}
+long c4_shift(const Geom &g) { return (16 - c4_origin(g) % 16) % 16; }
+
+// vy-pass: tv (E0, E1, E2, N3) -> C4 at positions -2..N+1 (its x rows hold
+// exactly the E0 positions, so a tile's output is contiguous like its input)
 vlp::LineMap vy_map(const Geom &g) {
   const long E0 = g.n[0] + 4, E1 = g.n[1] + 4, E2 = g.n[2] + 4;
-  const long F0 = g.n[0] + 6, F1 = g.n[1] + 6, F2 = g.n[2] + 6;
+  const vlp::C4Pitch cp = vlp::c4_pitch(g.n[0], g.n[1], g.n[2]);
   vlp::LineMap q{};
   q.N = g.n[3];
   q.X0 = E0; q.X1 = E1; q.A = E2;
   q.is0 = 1; q.is1 = E0; q.ios = E0 * E1; q.ils = E0 * E1 * E2;
-  q.os0 = 1; q.os1 = F0; q.oos = F0 * F1; q.ols = F0 * F1 * F2;
+  q.os0 = 1; q.os1 = cp.s1; q.oos = cp.s2; q.ols = cp.s3;
   return q;
 }
 
 template <typename T> int run_spline_vxvy(vlp_ctx *c) {
   const Geom &g = c->g;
-  const long F0 = g.n[0] + 6, F1 = g.n[1] + 6, F2 = g.n[2] + 6;
-  T *c4 = (T *)c->c4 + 1 + F0 * (1 + F1 * (1 + F2 * 1));
+  T *c4 = (T *)c->c4 + c4_origin(g);
   if (c->mf_m[0]) {  // matrix products on the MFMA units
     vlp::launch_spline_mfma<T>((const T *)c->cxy, (T *)c->tv, vx_map(g), (const T *)c->mf_m[0],
                                vlp::spline_mfma_kp(g.n[2]), c->stream);
@@ -619,7 +629,7 @@ static int create_ctx(const vlp_config *cfg_in, int precision, int device, int s
   c->n_cxy = lag ? 0 : (N0 + 4) * (N1 + 4) * N2 * N3;
   c->n_tv = lag ? 0 : (N0 + 4) * (N1 + 4) * (N2 + 4) * N3;
   c->n_c4 = lag ? 0 : (N0 + 4) * (N1 + 4) * (N2 + 4) * (N3 + 4);
-  c->n_c4s = lag ? 0 : (N0 + 6) * (N1 + 6) * (N2 + 6) * (N3 + 6) + 64;
+  c->n_c4s = lag ? 0 : vlp::c4_elems(N0, N1, N2, N3);
   c->n_l2 = vlp::l2_partials(c->g);
   c->xy_groups = lag ? 0 : vlp::xy_fused_groups(c->g, c->esz);
   c->rho_outer = N3;
@@ -647,7 +657,10 @@ static int create_ctx(const vlp_config *cfg_in, int precision, int device, int s
   if (!lag) {
     ALLOC(c->cxy, c->n_cxy * c->esz);
     ALLOC(c->tv, c->n_tv * c->esz);
-    ALLOC(c->c4, c->n_c4s * c->esz);
+    // C4 starts c4_shift elements into its allocation, so that each line of
+    // the vy pass's output starts on a 64-byte boundary
+    ALLOC(c->c4_alloc, (c->n_c4s + 16) * c->esz);
+    c->c4 = (char *)c->c4_alloc + c4_shift(c->g) * c->esz;
   }
   ALLOC(c->rho_part, N0 * N1 * N3 * (c->xy_groups > 1 ? (size_t)c->xy_groups : 1) * sizeof(double));
   ALLOC(c->l2_part, (size_t)c->n_l2 * sizeof(double));
@@ -882,14 +895,15 @@ int vlp_get_buffer(vlp_ctx *c, int which, double *host, size_t n) {
       std::vector<double> full(c->n_c4s);
       rc = download(c, c->c4, c->n_c4s, full.data());
       if (rc) return rc;
-      const size_t F0 = c->cfg.nxmax[0] + 6, F1 = c->cfg.nxmax[1] + 6, F2 = c->cfg.nxmax[2] + 6;
-      const size_t E0 = F0 - 2, E1 = F1 - 2, E2 = F2 - 2, E3 = (size_t)c->cfg.nxmax[3] + 4;
+      const vlp::C4Pitch cp = vlp::c4_pitch(c->cfg.nxmax[0], c->cfg.nxmax[1], c->cfg.nxmax[2]);
+      const size_t E0 = (size_t)c->cfg.nxmax[0] + 4, E1 = (size_t)c->cfg.nxmax[1] + 4,
+                   E2 = (size_t)c->cfg.nxmax[2] + 4, E3 = (size_t)c->cfg.nxmax[3] + 4;
       size_t k = 0;
       for (size_t i3 = 0; i3 < E3; i3++)
         for (size_t i2 = 0; i2 < E2; i2++)
           for (size_t i1 = 0; i1 < E1; i1++)
             for (size_t i0 = 0; i0 < E0; i0++)
-              host[k++] = full[(i0 + 1) + F0 * ((i1 + 1) + F1 * ((i2 + 1) + F2 * (i3 + 1)))];
+              host[k++] = full[(i0 + 1) + cp.s1 * (i1 + 1) + cp.s2 * (i2 + 1) + cp.s3 * (i3 + 1)];
       return VLP_OK;
     }
     default: break;
@@ -1014,8 +1028,7 @@ int vlp_autotune(vlp_ctx *c, const char *csv_path) {
   {
     hipEvent_t a = nullptr, b = nullptr;
     bool sok = hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess;
-    const long F0 = c->g.n[0] + 6, F1 = c->g.n[1] + 6, F2 = c->g.n[2] + 6;
-    const size_t c4off = 1 + F0 * (1 + F1 * (1 + F2 * 1));
+    const size_t c4off = c4_origin(c->g);
     for (auto &k : spl) {
       if (!sok) break;
       const vlp::LineMap m = k.pass ? vy_map(c->g) : vx_map(c->g);

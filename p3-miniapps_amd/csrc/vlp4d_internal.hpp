@@ -4,9 +4,17 @@
 //   F    : interior f, N0*N1*N2*N3                         (positions 0..N-1 per dim)
 //   CXY  : (N0+4)*(N1+4)*N2*N3, x,y on -2..N+1, v interior  (spline along x and y)
 //   TV   : (N0+4)*(N1+4)*(N2+4)*N3, also holds the x-pass output (N0+4)*N1*N2*N3
-//   C4   : (N0+6)*(N1+6)*(N2+6)*(N3+6), positions -3..N+2 in every dim; the
-//          coefficients fill -2..N+1 and the outer layer stays 0, so every
-//          stencil read of the 4-D advection is in bounds without clamping.
+//   C4   : positions -3..N+2 along y, vx, vy (the coefficients fill -2..N+1,
+//          the outer layer stays 0) and -2..N+1 along x, in rows of N0+4:
+//          position (p0, p1, p2, p3) at (p0+3) + s1 (p1+3) + s2 (p2+3) +
+//          s3 (p3+3) with the pitches of c4_pitch.  The x layer -3 / N0+2
+//          has no storage of its own (it overlaps the neighbouring row): no
+//          foot reads it with a nonzero weight, since feet stay within
+//          [x_min - dx, x_max] and the weight of point ip+2 vanishes at a
+//          foot on x_max.  Rows without x padding make the coefficients of
+//          one (vx, vy) contiguous, so the vy spline pass writes whole
+//          64-byte blocks (planes padded to 16 elements, base aligned by
+//          the library).
 // The reference's (N+6)^4 halo arrays of f are never materialised: the
 // periodic copies and alpha-sums of its halo exchange are computed inside the
 // spline kernels from the interior line they extend.
@@ -20,6 +28,17 @@
 #include "../../include/vlp4d.h"
 
 namespace vlp {
+
+// Pitches of the C4 storage (elements): row (y), plane (vx), volume (vy).
+struct C4Pitch {
+  long s1, s2, s3;
+};
+__host__ __device__ inline C4Pitch c4_pitch(long N0, long N1, long N2) {
+  const long s1 = N0 + 4, s2 = (s1 * (N1 + 6) + 1 + 15) / 16 * 16;
+  return C4Pitch{s1, s2, s2 * (N2 + 6)};
+}
+// Elements of the C4 storage including a zeroed tail.
+inline long c4_elems(long N0, long N1, long N2, long N3) { return c4_pitch(N0, N1, N2).s3 * (N3 + 6) + 64; }
 
 // Host helpers of vlp4d_api.cpp: record an error for vlp_last_error(NULL)
 // (returns code); fill dx from the box and check the configuration the way
