@@ -8,10 +8,10 @@
 //          the outer layer stays 0) and -2..N+1 along x, in rows of N0+4:
 //          position (p0, p1, p2, p3) at (p0+3) + s1 (p1+3) + s2 (p2+3) +
 //          s3 (p3+3) with the pitches of c4_pitch.  The x layer -3 / N0+2
-//          has no storage of its own (it overlaps the neighbouring row): no
-//          foot reads it with a nonzero weight, since feet stay within
-//          [x_min - dx, x_max] and the weight of point ip+2 vanishes at a
-//          foot on x_max.  Rows without x padding make the coefficients of
+//          has no storage of its own (it overlaps the neighbouring row):
+//          feet stay within [x_min - dx, x_max], so a stencil reaches N0+2
+//          only from a foot on x_max, where that point's weight is 0, and
+//          -3 only through rounding at x_min - dx, with a weight ~1e-16.  Rows without x padding make the coefficients of
 //          one (vx, vy) contiguous, so the vy spline pass writes whole
 //          64-byte blocks (planes padded to 16 elements, base aligned by
 //          the library).
@@ -34,6 +34,7 @@ struct C4Pitch {
   long s1, s2, s3;
 };
 __host__ __device__ inline C4Pitch c4_pitch(long N0, long N1, long N2) {
+  // + 1: the last row's N0+2 slot stays inside the plane's padding
   const long s1 = N0 + 4, s2 = (s1 * (N1 + 6) + 1 + 15) / 16 * 16;
   return C4Pitch{s1, s2, s2 * (N2 + 6)};
 }
