@@ -148,3 +148,25 @@ def test_bench_refuses_more_than_one_gpu():
     p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert p.returncode == 2 and "one GPU" in p.stderr
+
+
+def test_committed_traffic_summaries_parse():
+    """bench.py reads each kernel's measured HBM bytes per launch from the
+    committed rocprofv3 summaries; every entry parses and lies between the
+    kernel's algorithmic bytes (a lower bound on what it must move) and
+    four times that."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    n = 128
+    esz = {64: 8, 33: 4, 32: 4}
+    for prec, b in esz.items():
+        t = bench.measured_traffic("k_advect4d_ys", prec, n)
+        alg = bench.advect4d_bytes(n, b)
+        assert t is not None and alg <= t <= 4 * alg, (prec, t, alg)
+    # the Lagrange passes each stream f once: 16 bytes per cell in FP64
+    for kern in ("k_lag_xy", "k_lag_v"):
+        t = bench.measured_traffic(kern, 64, n, bench.LAG_TRAFFIC_FILE)
+        assert t is not None and 16 * n ** 4 <= t <= 4 * 16 * n ** 4, (kern, t)
+    assert bench.measured_traffic("k_advect4d_ys", 64, 96) is None
